@@ -1,0 +1,190 @@
+"""Benchmark: pair-interactions/s of the 3D WCSPH dam break on one MI355X.
+
+Workload (default): 3D dam break, dr = 0.0079, 4,096,766 fluid + 1,655,459
+wall particles, neighbour radius R = 4.8 dr (the Wendland support 2 hs,
+hs = 2.4 dr), FP32-mixed pair arithmetic with FP64 state.  A step is one
+fused device step: cell-list build, density sum, EOS, internal force with
+artificial viscosity, symplectic Euler kick and drift.
+
+  value = 2 * (accepted ordered pairs per step) * steps / wall time
+          (two pair passes per step: density, force), inputs resident in HBM.
+
+The same workload in plain FP64 (the drop-in path) is timed beside it, and
+the FP64 CPU oracle (a C port of the reference algorithm, OpenMP over
+particles) is timed on one step of the same workload on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "smoothedparticles.jl_amd"))
+
+# Published peaks of one MI355X.
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+# Algorithmic work of one force pass (per particle / per candidate / per
+# accepted pair): bytes read and written once in sorted order, 9 flops per
+# candidate test, about 45 flops per accepted force pair.
+FORCE_BYTES_PER_PARTICLE = {"mixed": 48.0, "fp64": 92.0}
+FLOPS_PER_CANDIDATE = 9.0
+FLOPS_PER_FORCE_PAIR = 45.0
+
+WORKLOADS = {
+    "dambreak3d_4M": dict(dim=3, dr=0.0079, R_over_dr=4.8,
+                          desc="3D WCSPH dam break, dr=0.0079, R=4.8dr"),
+    "dambreak3d_1M": dict(dim=3, dr=0.0125, R_over_dr=4.8,
+                          desc="3D WCSPH dam break, dr=0.0125, R=4.8dr"),
+    "dambreak2d": dict(dim=2, dr=0.02, R_over_dr=4.8,
+                       desc="2D WCSPH dam break, dr=0.02, R=4.8dr"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(st, precision, steps, warmup, autotune=True):
+    import smoothedparticles_hip as sp
+
+    p = st.params()
+    g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, precision)
+    g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
+    sp.create_cell_list(g)
+    T = g.autotune(**p) if autotune else None
+    if warmup:
+        sp.step(g, warmup, **p)
+    g.profile(True)
+    t0 = time.perf_counter()
+    pairs = sp.step(g, steps, **p)  # returns after a stream sync
+    t1 = time.perf_counter()
+    dens_pairs, force_pairs = g.step_pairs()
+    force_ms, force_n = g.profile_read(sp._native.SPH_KERNEL_FORCE)
+    dens_ms, dens_n = g.profile_read(sp._native.SPH_KERNEL_DENSITY)
+    build_ms, build_n = g.profile_read(sp._native.SPH_KERNEL_BUILD)
+    g.profile(False)
+    sp.create_cell_list(g)
+    cand_all, cand_fluid = g.candidates()
+    n = len(g)
+    g.close()
+    el = t1 - t0
+    return dict(
+        elapsed=el, pairs=pairs, value=2.0 * pairs / el, n=n, tile=T,
+        force_ms=force_ms / max(force_n, 1), density_ms=dens_ms / max(dens_n, 1),
+        build_ms=build_ms / max(build_n, 1), force_pairs_per_pass=force_pairs / steps,
+        cand_fluid=cand_fluid, cand_all=cand_all,
+    )
+
+
+def cpu_baseline(st):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    o = O.OracleSystem(st.R, st.bbox_lo, st.bbox_hi)
+    o.load(st.x, st.v, st.rho, st.P, st.type)
+    t0 = time.perf_counter()
+    pairs = o.step_wcsph(1, **st.params())
+    el = time.perf_counter() - t0
+    return dict(value=2.0 * pairs / el, unit="pair-interactions/s", cores=O.num_threads(),
+                kind="port",
+                sample=f"1 full step (cell list, density, EOS, force, integrate) of the same "
+                       f"{len(st.x)}-particle workload by the FP64 C port of the reference "
+                       f"algorithm, OpenMP over particles; {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="dambreak3d_4M", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(each system runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from smoothedparticles_hip import states
+
+    w = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    st = states.dam_break(w["dim"], w["dr"], R_over_dr=w["R_over_dr"])
+    log(f"state: {len(st.x)} particles ({st.n_fluid} fluid) in {time.perf_counter() - t0:.1f} s")
+
+    res = run_gpu(st, args.precision, args.steps, args.warmup)
+    log(f"{args.precision}: {res['value']:.4g} pair-int/s, {1e3 * res['elapsed'] / args.steps:.2f} ms/step, "
+        f"force {res['force_ms']:.3f} ms, density {res['density_ms']:.3f} ms, build {res['build_ms']:.3f} ms, T={res['tile']}")
+    other = None
+    if not args.no_fp64 and args.precision == "mixed":
+        other = run_gpu(st, "fp64", args.steps, args.warmup)
+        log(f"fp64: {other['value']:.4g} pair-int/s, force {other['force_ms']:.3f} ms, T={other['tile']}")
+
+    prec = args.precision
+    peak = PEAK_FP32_TFLOPS if prec == "mixed" else PEAK_FP64_TFLOPS
+    flops = FLOPS_PER_CANDIDATE * res["cand_fluid"] + FLOPS_PER_FORCE_PAIR * res["force_pairs_per_pass"]
+    achieved = flops / (res["force_ms"] * 1e-3) / 1e12
+    hbm_bytes = FORCE_BYTES_PER_PARTICLE[prec] * res["n"]
+    hbm_gbs = hbm_bytes / (res["force_ms"] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"force_pass_{args.workload}_{prec}_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    out = {
+        "metric": "pair-interactions/s, FP32-mixed vs FP64 plain at stated error; p50/p99 latency",
+        "value": res["value"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * res["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed" if prec == "mixed" else "f64",
+        "data": "synthetic (seeded lattice dam break)",
+        "config": {
+            "workload": w["desc"] + f", {res['n']} particles ({st.n_fluid} fluid), "
+                        f"{'FP32-mixed pair arithmetic, FP64 state' if prec == 'mixed' else 'plain FP64'}",
+            "particles": res["n"],
+            "neighbour_radius": st.R,
+            "pairs_per_step": res["pairs"] / args.steps,
+            "tile_targets": res["tile"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "ceiling": "vector ALU: the force kernel issues no MFMA; peak is the published "
+                       f"{'FP32' if prec == 'mixed' else 'FP64'} rate",
+            "kernel": "force pass (pressure gradient + artificial viscosity)",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "hbm_algorithmic_GBps": hbm_gbs,
+            "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
+            "avg_launch_ms": res["force_ms"],
+        },
+    }
+    if other is not None:
+        out["fp64_value"] = other["value"]
+        out["mixed_over_fp64"] = res["value"] / other["value"]
+        out["fp64_ms_per_step"] = 1e3 * other["elapsed"] / args.steps
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(st)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

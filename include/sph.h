@@ -1,0 +1,196 @@
+/*
+ * sph.h -- C ABI of the MI355X (gfx950) cell-list neighbour search and
+ * pair-interaction library.  This is the drop-in boundary for the hot path
+ * of SmoothedParticles.jl v0.2.0:
+ *
+ *   ParticleSystem(T, domain, h)      src/structs.jl:57-92   -> sph_create
+ *   push!(sys.particles, ...)         src/grids.jl:253-258   -> sph_append
+ *   ParticleField(sys, :sym)          src/structs.jl:118-125 -> sph_get_field / sph_set_field
+ *   create_cell_list!(sys)            src/core.jl:51-90      -> sph_build_cells
+ *   apply_binary!(sys, op!)           src/core.jl:125-129    -> sph_apply_binary
+ *   apply_unary!(sys, op!)            src/core.jl:138-142    -> sph_apply_unary
+ *   apply!(sys, op!; self)            src/core.jl:151-161    -> sph_apply
+ *   sum(p -> energy(p), particles)    tests/test_collision_2d.jl:137,
+ *                                     examples/collapse_dry.jl:173-187 -> sph_reduce
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative SPH_E* code on
+ *     failure; sph_last_error() then holds a message (per host thread).
+ *   - Host pointers are borrowed for the duration of the call only.  Device
+ *     memory is owned by the system handle.
+ *   - Calls are synchronous: they return after the device work finished,
+ *     as the reference's threaded loops do.
+ *   - A handle is not thread-safe; one host thread drives it at a time.
+ *   - Particle order seen through get/set/append is the reference's order of
+ *     sys.particles, including its swap-with-end removal of particles that
+ *     left the domain (src/core.jl:63-81).  The device keeps its own
+ *     cell-sorted order internally.
+ *   - Host layout of vector fields is [n][3] doubles (RealVector).
+ */
+#ifndef SPH_H
+#define SPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sph_system sph_system;
+
+/* error codes */
+#define SPH_OK 0
+#define SPH_EINVAL -1   /* invalid argument */
+#define SPH_EHIP -2     /* HIP runtime error */
+#define SPH_ESTATE -3   /* call not valid in the current state */
+#define SPH_ENOMEM -4   /* device allocation failed */
+#define SPH_EUNSUP -5   /* unsupported operator or configuration */
+
+/* precision modes */
+#define SPH_FP64 0   /* plain FP64 everywhere (the drop-in) */
+#define SPH_MIXED 1  /* FP32 relative-coordinate pair arithmetic with compensated
+                        accumulation; FP64 positions, state and integrator */
+
+/* fields (sph_get_field / sph_set_field) */
+#define SPH_FIELD_X 0     /* position  [n][3] */
+#define SPH_FIELD_V 1     /* velocity  [n][3] */
+#define SPH_FIELD_A 2     /* acceleration (Dv) [n][3] */
+#define SPH_FIELD_RHO 3   /* density   [n] */
+#define SPH_FIELD_P 4     /* pressure  [n] */
+#define SPH_FIELD_DRHO 5  /* density rate [n] */
+#define SPH_FIELD_RHO0 6  /* per-particle reference density [n] */
+#define SPH_FIELD_TYPE 7  /* particle type, FLUID = 0.0 [n] */
+
+/* Binary operators, registered by the name of the reference operator they
+ * replace.  Each runs over the neighbour set of src/core.jl:94-112. */
+#define SPH_OP_DENSITY_SUM 1         /* find_rho!  tests/test_collision_2d.jl:63-65:
+                                        rho += m*W(h,r) */
+#define SPH_OP_DENSITY_SUM_RHO0 2    /* find_rho0! tests/test_collision_2d.jl:67-69 */
+#define SPH_OP_INTERNAL_FORCE 3      /* internal_force! examples/collapse_dry.jl:135-141:
+                                        fluid targets, pressure gradient with
+                                        P/rho^2, laminar mu term, plus Monaghan
+                                        artificial viscosity alpha (not in the
+                                        reference; alpha = 0 disables it) */
+#define SPH_OP_PRESSURE_FORCE_RHO0 4 /* internal_force! tests/test_collision_2d.jl:75-78 */
+#define SPH_OP_BALANCE_OF_MASS 5     /* balance_of_mass! examples/collapse_dry.jl:112-115 */
+
+/* Unary operators */
+#define SPH_OP_RESET_RHO 16          /* reset_rho!  tests/test_collision_2d.jl:84-86 */
+#define SPH_OP_RESET_A 17            /* reset_a!    tests/test_collision_2d.jl:80-82 */
+#define SPH_OP_FIND_PRESSURE 18      /* P = c^2 (rho - rho0)  examples/collapse_dry.jl:126 */
+#define SPH_OP_FIND_PRESSURE_RHO0 19 /* P = c^2 (rho - rho0_p) tests/test_collision_2d.jl:71-73 */
+#define SPH_OP_FIND_PRESSURE_CONTINUITY 20 /* find_pressure! examples/collapse_dry.jl:123-127 */
+#define SPH_OP_ACCELERATE 21         /* accelerate! fluid: v += (coeff*dt)(a + g)
+                                        examples/collapse_dry.jl:155-159 */
+#define SPH_OP_MOVE 22               /* move! fluid: x += (coeff*dt) v
+                                        examples/collapse_dry.jl:148-153 */
+
+/* Reductions (sph_reduce) */
+#define SPH_REDUCE_ENERGY 0           /* sum of energy(p), examples/collapse_dry.jl:166-171 (1 value) */
+#define SPH_REDUCE_ENERGY_COLLISION 1 /* tests/test_collision_2d.jl:96-100 (1 value) */
+#define SPH_REDUCE_MOMENTUM 2         /* sum of m v (3 values) */
+
+/* Fused step schemes (sph_step) */
+#define SPH_SCHEME_WCSPH_SE 0  /* build cells; rho = sum m W (self); EOS; a = internal
+                                  force; v += dt (a + g); x += dt v  (fluid only) */
+
+typedef struct sph_params {
+    double m;      /* particle mass */
+    double rho0;   /* reference density */
+    double c;      /* speed of sound */
+    double alpha;  /* artificial viscosity coefficient (0 = off) */
+    double hs;     /* smoothing length of the viscosity term */
+    double mu;     /* laminar viscosity (0 = off) */
+    double nu;     /* density diffusion of balance_of_mass! */
+    double g[3];   /* gravity */
+    double dt;     /* time step */
+    double coeff;  /* multiplier of dt in MOVE / ACCELERATE */
+} sph_params;
+
+/* ParticleSystem(T, domain, h), src/structs.jl:57-92.  The domain is given
+ * by its bounding box, which is all the reference keeps of it (:63).
+ * h > 0 is the neighbour radius and cell size.  The system is 2D when
+ * floor(max_z/h) == floor(min_z/h), as in the reference (:70). */
+int sph_create(sph_system** out, double h, const double bbox_min[3],
+               const double bbox_max[3], int precision);
+int sph_destroy(sph_system* s);
+
+/* push! of n particles at the end of sys.particles (generate_particles!,
+ * src/grids.jl:253-258).  x is [n][3]; v may be NULL (zero); rho, P, type
+ * may be NULL (zero).  Other fields start at zero. */
+int sph_append(sph_system* s, int64_t n, const double* x, const double* v,
+               const double* rho, const double* P, const double* type);
+
+/* length(sys.particles) */
+int sph_count(sph_system* s, int64_t* n);
+
+/* ParticleField get/set in reference order; n must equal sph_count. */
+int sph_set_field(sph_system* s, int field, const double* host, int64_t n);
+int sph_get_field(sph_system* s, int field, double* host, int64_t n);
+
+/* create_cell_list!(sys), src/core.jl:51-90: removes particles outside the
+ * bounding box (swap-with-end, as the reference) and builds the cell list.
+ * *n_out (may be NULL) receives the new particle count. */
+int sph_build_cells(sph_system* s, int64_t* n_out);
+
+/* apply_binary!(sys, op) and, if self != 0, op(p, p, 0.0) afterwards
+ * (src/core.jl:151-157).  Requires a cell list built since the last change
+ * of positions or particle count. */
+int sph_apply_binary(sph_system* s, int op, const sph_params* params, int self);
+/* apply_unary!(sys, op) */
+int sph_apply_unary(sph_system* s, int op, const sph_params* params);
+/* apply!(sys, op; self): dispatches on the operator id. */
+int sph_apply(sph_system* s, int op, const sph_params* params, int self);
+
+/* Global reductions; out receives 1 or 3 doubles. */
+int sph_reduce(sph_system* s, int what, const sph_params* params, double* out);
+
+/* Neighbour pairs of the current cell list as reference particle indices
+ * (p, q), p != q, dist <= h: pairs[2k], pairs[2k+1].  The order is
+ * unspecified.  *n receives the total; at most cap pairs are written. */
+int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n);
+
+/* nsteps fused steps of a scheme, device-resident, one host sync at the end.
+ * *pairs (may be NULL) receives the accepted ordered pairs summed over the
+ * density passes of all steps. */
+int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps,
+             int64_t* pairs);
+
+/* Accepted ordered pairs of the last sph_step call, summed over its steps:
+ * density passes (all targets) and force passes (fluid targets only). */
+int sph_step_pairs(sph_system* s, int64_t* density_pairs, int64_t* force_pairs);
+
+/* Cell geometry as the reference computes it (src/structs.jl:63-82):
+ * key_phase[3], key_lim[3], key_max, key_diff[27] (9 used in 2D). */
+int sph_key_geometry(sph_system* s, int64_t* key_phase, int64_t* key_lim,
+                     int64_t* key_max, int64_t* key_diff, int* n_diff);
+
+/* Targets per tile of the pair kernels (8, 16, 32 or 64; 0 = automatic).
+ * sph_autotune times the density kernel for each tile size on the current
+ * cell list and keeps the fastest; *best receives it. */
+int sph_set_tile(sph_system* s, int targets_per_tile);
+int sph_autotune(sph_system* s, const sph_params* params, int* best);
+
+/* Diagnostics (no counterpart in the reference): candidate particles the
+ * pair kernels test per pass under the current cell list, i.e. for every
+ * particle the particles of the cells key + key_diff it visits
+ * (src/core.jl:96-102); *all over all targets, *fluid over fluid targets. */
+int sph_candidates(sph_system* s, int64_t* all, int64_t* fluid);
+
+/* Kernel timing with HIP events on the system's stream.  Kernel ids:
+ * 0 cell build, 1 density pass, 2 force pass, 3 unary/integrator. */
+#define SPH_KERNEL_BUILD 0
+#define SPH_KERNEL_DENSITY 1
+#define SPH_KERNEL_FORCE 2
+#define SPH_KERNEL_UNARY 3
+int sph_profile(sph_system* s, int enable);
+int sph_profile_read(sph_system* s, int kernel, double* total_ms, int64_t* launches);
+
+/* Message of the last failing call on this host thread. */
+const char* sph_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_H */

@@ -1,0 +1,319 @@
+// Cell-list builder: the device form of create_cell_list! (src/core.jl:51-90).
+//
+// Stages, all on the system's stream, no host round trip:
+//   1. k_keys     FP64 find_key (src/structs.jl:97-106) and the inclusive
+//                 bounding-box test (src/geometry.jl:24-30); particles
+//                 outside get the sentinel key key_max.  Per-cell counts by
+//                 wave-aggregated atomics (one atomic per distinct key in a
+//                 wave; the input is the previous sorted order, so a wave
+//                 usually touches one or two cells).
+//   2. scan       exclusive prefix sum of the counts -> cell_start
+//   3. k_scatter  bucket every particle into its cell range
+//   4. k_rank     order each cell by descending reference index, the order
+//                 add_index! keeps (src/core.jl:26-41); deterministic
+//                 whatever order the atomics handed out
+//   5. k_gather   permute every column into the new cell-sorted order
+//   6. removal    particles outside the box sit in the sentinel bucket at
+//                 the end, already sorted by descending reference index; the
+//                 reference's swap-with-end renumbering (src/core.jl:72-81)
+//                 is replayed on their reference indices
+//   7. tiles      a scan of ceil(count/T) per cell and the tile table of the
+//                 pair kernels: each tile is up to T targets of one cell
+//
+// This is a counting sort on the cell key (one radix digit of
+// ceil(log2(key_max+1)) bits) followed by an in-bucket rank.
+#include "sph_internal.h"
+
+namespace sph {
+
+static constexpr int SCAN_ITEMS = 16;
+static constexpr int SCAN_TILE = 256 * SCAN_ITEMS;
+
+__global__ void k_fill_u32(unsigned* p, unsigned v, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ keys --
+__global__ __launch_bounds__(256) void k_keys(Geo g, const double* __restrict__ x,
+                                              const double* __restrict__ y,
+                                              const double* __restrict__ z, int* meta,
+                                              int* __restrict__ key, int* __restrict__ slot,
+                                              unsigned* __restrict__ cnt)
+{
+    const int n = meta[META_N];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[META_NTOT] = n;
+    int k = -1;
+    if (i < n) {
+        const double px = x[i], py = y[i], pz = z[i];
+        const bool inside = g.bmin[0] <= px && px <= g.bmax[0] && g.bmin[1] <= py &&
+                            py <= g.bmax[1] && g.bmin[2] <= pz && pz <= g.bmax[2];
+        if (inside) {
+            long long ci = (long long)floor(px / g.h) - g.kp[0];
+            long long cj = (long long)floor(py / g.h) - g.kp[1];
+            long long ck = (long long)floor(pz / g.h) - g.kp[2];
+            // In range by monotonicity of the correctly rounded division;
+            // the clamp only guards memory.
+            ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
+            cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
+            ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
+            k = (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+        } else {
+            k = g.key_max;
+        }
+        key[i] = k;
+    }
+    // One atomic per distinct key among the wave's lanes.
+    const int lane = __lane_id();
+    unsigned long long active = __ballot(i < n);
+    unsigned myslot = 0;
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int lk = __shfl(k, leader);
+        const unsigned long long peers = __ballot(k == lk) & active;
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[lk], (unsigned)__popcll(peers));
+        base = __shfl(base, leader);
+        if ((peers >> lane) & 1ull)
+            myslot = base + (unsigned)__popcll(peers & ((1ull << lane) - 1ull));
+        active &= ~peers;
+    }
+    if (i < n) slot[i] = (int)myslot;
+}
+
+// ------------------------------------------------------------------ scan --
+__device__ inline unsigned wave_incl_scan(unsigned v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// Inclusive scan over a 256-thread block; `total` gets the block sum.
+__device__ inline unsigned block_incl_scan(unsigned v, unsigned& total)
+{
+    __shared__ unsigned wsum[4];
+    const unsigned w = wave_incl_scan(v);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) wsum[wid] = w;
+    __syncthreads();
+    unsigned off = 0;
+    for (int i = 0; i < wid; ++i) off += wsum[i];
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return w + off;
+}
+
+struct SrcCount {
+    const unsigned* c;
+    __device__ unsigned operator()(int i) const { return c[i]; }
+};
+struct SrcTiles {
+    const int* cs;
+    int km;
+    int T;
+    __device__ unsigned operator()(int i) const
+    {
+        if (i >= km) return 0u;
+        const int n = cs[i + 1] - cs[i];
+        return (unsigned)((n + T - 1) / T);
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(256) void k_scan_reduce(Src src, int n, unsigned* part)
+{
+    const int base = blockIdx.x * SCAN_TILE;
+    unsigned s = 0;
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int i = base + it * 256 + threadIdx.x;
+        if (i < n) s += src(i);
+    }
+    unsigned total;
+    block_incl_scan(s, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partials(unsigned* part, int nb)
+{
+    unsigned carry = 0;
+    for (int b = 0; b < nb; b += 256) {
+        const int i = b + threadIdx.x;
+        const unsigned v = i < nb ? part[i] : 0u;
+        unsigned total;
+        const unsigned inc = block_incl_scan(v, total);
+        if (i < nb) part[i] = carry + inc - v;
+        carry += total;
+    }
+}
+
+// out[i] = exclusive prefix for i < n, out[n] = total.
+template <class Src>
+__global__ __launch_bounds__(256) void k_scan_down(Src src, int n, const unsigned* part,
+                                                   int* out)
+{
+    const int base = blockIdx.x * SCAN_TILE;
+    unsigned carry = part[blockIdx.x];
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int i = base + it * 256 + threadIdx.x;
+        const unsigned v = i < n ? src(i) : 0u;
+        unsigned total;
+        const unsigned inc = block_incl_scan(v, total);
+        if (i < n) out[i] = (int)(carry + inc - v);
+        if (i == n - 1) out[n] = (int)(carry + inc);
+        carry += total;
+    }
+}
+
+template <class Src>
+static void scan(sph_system* s, Src src, int n, int* out)
+{
+    const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan_reduce<<<nb, 256, 0, s->stream>>>(src, n, s->scan_part);
+    k_scan_partials<<<1, 256, 0, s->stream>>>(s->scan_part, nb);
+    k_scan_down<<<nb, 256, 0, s->stream>>>(src, n, s->scan_part, out);
+}
+
+long long scan_partials_needed(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// -------------------------------------------------- scatter, rank, gather --
+__global__ __launch_bounds__(256) void k_scatter(const int* meta, const int* __restrict__ key,
+                                                 const int* __restrict__ slot,
+                                                 const int* __restrict__ ref,
+                                                 const int* __restrict__ cell_start,
+                                                 int* tmp_idx, int* tmp_key, int* tmp_ref)
+{
+    const int n = meta[META_NTOT];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int k = key[i];
+    const int pos = cell_start[k] + slot[i];
+    tmp_idx[pos] = i;
+    tmp_key[pos] = k;
+    tmp_ref[pos] = ref[i];
+}
+
+__global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __restrict__ cell_start,
+                                              const int* __restrict__ tmp_idx,
+                                              const int* __restrict__ tmp_key,
+                                              const int* __restrict__ tmp_ref, int* perm)
+{
+    const int n = meta[META_NTOT];
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= n) return;
+    const int c = tmp_key[u];
+    const int s = cell_start[c], e = cell_start[c + 1];
+    const int r = tmp_ref[u];
+    int rank = 0;
+    for (int v = s; v < e; ++v) rank += tmp_ref[v] > r;
+    perm[s + rank] = tmp_idx[u];
+}
+
+__global__ __launch_bounds__(256) void k_gather(const int* meta, const int* __restrict__ perm,
+                                                const double* __restrict__ src,
+                                                double* __restrict__ dst,
+                                                const int* __restrict__ isrc,
+                                                int* __restrict__ idst, long long cap)
+{
+    const int n = meta[META_NTOT];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const long long j = perm[k];
+#pragma unroll
+    for (int f = 0; f < NCOL; ++f) dst[f * cap + k] = src[f * cap + j];
+#pragma unroll
+    for (int f = 0; f < NICOL; ++f) idst[f * cap + k] = isrc[f * cap + j];
+}
+
+__global__ void k_finalize(int* meta, const int* cell_start, int key_max)
+{
+    const int ntot = meta[META_NTOT];
+    const int na = cell_start[key_max];
+    meta[META_NREM] = ntot - na;
+    meta[META_N] = na;
+    meta[META_NCELLS] = 0;
+}
+
+// Removal renumbering (src/core.jl:72-81): with the removed reference
+// indices r_1 > r_2 > ... > r_m, particles[r_i] = particles[N+1-i] in turn.
+// inv maps a reference position to the sorted slot that holds it.
+__global__ __launch_bounds__(256) void k_inv(const int* meta, const int* __restrict__ ref,
+                                             int* inv)
+{
+    if (meta[META_NREM] == 0) return;
+    const int n = meta[META_NTOT];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) inv[ref[k]] = k;
+}
+
+__global__ void k_remove_seq(const int* meta, int* ref, int* inv)
+{
+    const int m = meta[META_NREM];
+    if (m == 0) return;
+    const int na = meta[META_N], ntot = meta[META_NTOT];
+    for (int i = 0; i < m; ++i) {
+        const int r = ref[na + i];     // descending
+        const int src = ntot - 1 - i;  // reference position N+1-i (1-based)
+        const int slot = inv[src];
+        inv[r] = slot;
+        ref[slot] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_start,
+                                               const int* __restrict__ tile_start, int key_max,
+                                               int T, int2* tiles, int* meta)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c == 0) meta[META_NTILES] = tile_start[key_max];
+    if (c >= key_max) return;
+    const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
+    if (nt > 0) atomicAdd(&meta[META_NCELLS], 1);
+    const int cs = cell_start[c];
+    for (int t = 0; t < nt; ++t) tiles[t0 + t] = make_int2(c, cs + t * T);
+}
+
+// ------------------------------------------------------------ host side --
+static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void enqueue_tiles(sph_system* s)
+{
+    const Geo& g = s->geo;
+    scan(s, SrcTiles{s->cell_start, g.key_max, s->tile_T}, g.key_max + 1, s->tile_start);
+    k_tiles<<<nblocks(g.key_max + 1, 256), 256, 0, s->stream>>>(
+        s->cell_start, s->tile_start, g.key_max, s->tile_T, s->tiles, s->meta);
+}
+
+void enqueue_build(sph_system* s)
+{
+    const Geo& g = s->geo;
+    hipStream_t st = s->stream;
+    const long long cap = s->cap;
+    const unsigned nb = nblocks(s->n > 0 ? s->n : 1, 256);
+    double* c = s->cols[s->cur];
+    int* ic = s->icols[s->cur];
+    k_fill_u32<<<nblocks(g.key_max + 1, 256), 256, 0, st>>>(s->cnt, 0u, g.key_max + 1);
+    k_keys<<<nb, 256, 0, st>>>(g, c + C_X * cap, c + C_Y * cap, c + C_Z * cap, s->meta,
+                               ic + I_KEY * cap, s->slot, s->cnt);
+    scan(s, SrcCount{s->cnt}, g.key_max + 1, s->cell_start);
+    k_scatter<<<nb, 256, 0, st>>>(s->meta, ic + I_KEY * cap, s->slot, ic + I_REF * cap,
+                                  s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref);
+    k_rank<<<nb, 256, 0, st>>>(s->meta, s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref,
+                               s->perm);
+    k_gather<<<nb, 256, 0, st>>>(s->meta, s->perm, c, s->cols[s->cur ^ 1], ic,
+                                 s->icols[s->cur ^ 1], cap);
+    s->cur ^= 1;
+    int* ref = s->icols[s->cur] + I_REF * cap;
+    k_finalize<<<1, 1, 0, st>>>(s->meta, s->cell_start, g.key_max);
+    k_inv<<<nb, 256, 0, st>>>(s->meta, ref, s->inv);
+    k_remove_seq<<<1, 1, 0, st>>>(s->meta, ref, s->inv);
+    enqueue_tiles(s);
+}
+
+}  // namespace sph

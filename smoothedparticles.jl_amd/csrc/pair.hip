@@ -1,0 +1,770 @@
+// Pair-interaction kernels: the device form of _apply_binary! /
+// apply_binary! (src/core.jl:94-129) for the registered operators.
+//
+// Work decomposition.  A tile is up to T targets of one cell (T = 8..64).
+// One 64-lane wave handles a tile: lane = (target t, slice s), t = lane % T,
+// s = lane / T, with S = 64 / T slices.  The candidates of a target are the
+// cells key + key_diff of the reference; with di innermost, each of the 3
+// (2D) or 9 (3D) (dj, dk) rows is the contiguous cell range
+// [key + L1*dj + L1*L2*dk - 1, ... + 1], clamped to [0, key_max) exactly as
+// the reference's linear bounds test (src/core.jl:98), so rows wrap and
+// small grids visit aliased cells as often as the reference does.  Since
+// the particles are cell-sorted, each row is one contiguous particle range.
+// The wave streams it through LDS in chunks of 64 (one coalesced load per
+// lane), every lane then tests the chunk entries of its slice, and at the
+// end the S slices of a target are combined with wave shuffles.
+//
+// Predicate (src/core.jl:104-105, src/algebra.jl:49-60): accept q != p when
+// RN(sqrt((d1*d1 + d2*d2) + d3*d3)) <= h.  sqrt is correctly rounded and
+// monotonic, so this equals s <= s_thr with s_thr = max{s : RN(sqrt(s)) <= h}
+// (computed on the host).  The FP64 kernels evaluate s without FMA
+// (-ffp-contract=off), i.e. bit-exactly as the reference.  The mixed
+// kernels evaluate s in FP32 on tile-relative coordinates and fall back to
+// the exact FP64 test inside a guard band around s_thr, so their neighbour
+// sets are the reference's too.
+#include "sph_internal.h"
+
+namespace sph {
+
+#define COL(c, f, i) ((c).cols[(long long)(f) * (c).cap + (i)])
+
+// Guard band of the mixed predicate, relative to s_thr.  The FP32 squared
+// distance of tile-relative coordinates (|coords| <= 2*sqrt(3)*h) is within
+// ~2e-6 * h^2 of the exact one; the band is 5x wider.
+static constexpr double BAND = 1.0e-5;
+
+__device__ inline void count_pairs(const PairCtx& c, unsigned n)
+{
+    // wave reduction of the per-lane counts, one atomic per wave
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o);
+    if (__lane_id() == 0 && n) atomicAdd(c.pair_counter, (unsigned long long)n);
+}
+
+// Compensated (2Sum) addition of a chunk partial into (s, e).
+__device__ inline void two_sum_add(float& s, float& e, float x)
+{
+    const float t = s + x;
+    const float bp = t - s;
+    e += (s - (t - bp)) + (x - bp);
+    s = t;
+}
+
+// ================================================================ FP64 ====
+
+struct LdsPos64 {
+    double x[64], y[64], z[64];
+};
+struct LdsForce64 {
+    double x[64], y[64], z[64], vx[64], vy[64], vz[64], pr[64], rho[64];
+};
+struct LdsCont64 {
+    double x[64], y[64], z[64], vx[64], vy[64], vz[64], rho[64];
+};
+struct LdsPres64 {
+    double x[64], y[64], z[64], P[64];
+};
+
+template <class L>
+__device__ inline void stage_pos64(const PairCtx& c, int q, int l, L& s)
+{
+    s.x[l] = COL(c, C_X, q);
+    s.y[l] = COL(c, C_Y, q);
+    s.z[l] = COL(c, C_Z, q);
+}
+
+// rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0); also the
+// accepted-pair counter of the metric.
+template <int OUT>
+struct Density64 {
+    using Lds = LdsPos64;
+    struct Tgt {
+        double x, y, z;
+    };
+    struct Acc {
+        double w;
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s) { stage_pos64(c, q, l, s); }
+    static __device__ void init(Acc& a) { a.w = 0.0; a.n = 0; }
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int,
+                                    double, double, double, double ss, Acc& a)
+    {
+        const double r = sqrt(ss);
+        const double x = r * c.g.inv_h;
+        const double u = 1.0 - x;
+        const double u2 = u * u;
+        a.w += u2 * u2 * (1.0 + 4.0 * x);
+        a.n += 1;
+    }
+    static __device__ void reduce(Acc& a, int T)
+    {
+        for (int o = T; o < 64; o <<= 1) a.w += __shfl_xor(a.w, o);
+    }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
+    {
+        double v = c.overwrite ? 0.0 : COL(c, OUT, p);
+        v = v + c.ph.kW * a.w;
+        if (c.self) v = v + c.ph.kW;
+        COL(c, OUT, p) = v;
+    }
+    static __device__ unsigned pairs(const Acc& a) { return a.n; }
+};
+
+// internal_force! (examples/collapse_dry.jl:135-141) + Monaghan viscosity.
+// With u = 1 - r/h and rDW = C_dW u^3 / h^(d+2), kdW = m C_dW / h^(d+2):
+//   pressure  a += -kdW u^3 (P_p/rho_p^2 + P_q/rho_q^2) x_pq
+//   viscosity a += -kdW u^3 Pi x_pq,  Pi = avk (v.x)/(rhobar (r^2 + eps)), v.x < 0
+//   laminar   a += 2 kdW mu / rho0^2 u^3 v_pq
+struct Force64 {
+    using Lds = LdsForce64;
+    struct Tgt {
+        double x, y, z, vx, vy, vz, pr, rho;
+        bool fluid;
+    };
+    struct Acc {
+        double ax, ay, az, lx, ly, lz;
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
+        t.vx = COL(c, C_VX, p); t.vy = COL(c, C_VY, p); t.vz = COL(c, C_VZ, p);
+        t.pr = c.prho2[p];
+        t.rho = COL(c, C_RHO, p);
+        t.fluid = COL(c, C_TYPE, p) == 0.0;
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt& t) { return t.fluid; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    {
+        stage_pos64(c, q, l, s);
+        s.vx[l] = COL(c, C_VX, q); s.vy[l] = COL(c, C_VY, q); s.vz[l] = COL(c, C_VZ, q);
+        s.pr[l] = c.prho2[q];
+        s.rho[l] = COL(c, C_RHO, q);
+    }
+    static __device__ void init(Acc& a)
+    {
+        a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0;
+        a.n = 0;
+    }
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    {
+        const double r = sqrt(ss);
+        const double x = r * c.g.inv_h;
+        const double u = 1.0 - x;
+        const double u3 = u * u * u;
+        double A = t.pr + s.pr[jj];
+        const double dvx = t.vx - s.vx[jj], dvy = t.vy - s.vy[jj], dvz = t.vz - s.vz[jj];
+        if (c.ph.avk != 0.0) {
+            double vx = dvx * dx + dvy * dy;
+            vx = vx + dvz * dz;
+            if (vx < 0.0) {
+                const double rhobar = 0.5 * (t.rho + s.rho[jj]);
+                A = A + c.ph.avk * vx / (rhobar * (r * r + c.ph.eps));
+            }
+        }
+        const double f = u3 * A;
+        a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
+        if (c.ph.mu != 0.0) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
+        a.n += 1;
+    }
+    static __device__ void reduce(Acc& a, int T)
+    {
+        for (int o = T; o < 64; o <<= 1) {
+            a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
+            a.lx += __shfl_xor(a.lx, o); a.ly += __shfl_xor(a.ly, o); a.lz += __shfl_xor(a.lz, o);
+        }
+    }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
+    {
+        const double kp = -c.ph.kdW;
+        const double kl = 2.0 * c.ph.kdW * c.ph.mu / (c.ph.rho0 * c.ph.rho0);
+        double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
+        double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
+        double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
+        ax = ax + kp * a.ax; ay = ay + kp * a.ay; az = az + kp * a.az;
+        if (c.ph.mu != 0.0) { ax = ax + kl * a.lx; ay = ay + kl * a.ly; az = az + kl * a.lz; }
+        COL(c, C_AX, p) = ax; COL(c, C_AY, p) = ay; COL(c, C_AZ, p) = az;
+    }
+    static __device__ unsigned pairs(const Acc& a) { return a.n; }
+};
+
+// internal_force! of tests/test_collision_2d.jl:75-78 (P/rho0^2, all targets).
+struct ForceRho064 {
+    using Lds = LdsPres64;
+    struct Tgt {
+        double x, y, z, P;
+    };
+    struct Acc {
+        double ax, ay, az;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p); t.P = COL(c, C_P, p);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    {
+        stage_pos64(c, q, l, s);
+        s.P[l] = COL(c, C_P, q);
+    }
+    static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    {
+        const double r = sqrt(ss);
+        const double u = 1.0 - r * c.g.inv_h;
+        const double f = u * u * u * (t.P + s.P[jj]);
+        a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
+    }
+    static __device__ void reduce(Acc& a, int T)
+    {
+        for (int o = T; o < 64; o <<= 1) {
+            a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
+        }
+    }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
+    {
+        const double k = -c.ph.kdW / (c.ph.rho0 * c.ph.rho0);
+        double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
+        double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
+        double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
+        COL(c, C_AX, p) = ax + k * a.ax;
+        COL(c, C_AY, p) = ay + k * a.ay;
+        COL(c, C_AZ, p) = az + k * a.az;
+    }
+    static __device__ unsigned pairs(const Acc&) { return 0; }
+};
+
+// balance_of_mass! (examples/collapse_dry.jl:112-115):
+//   Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q))
+struct Continuity64 {
+    using Lds = LdsCont64;
+    struct Tgt {
+        double x, y, z, vx, vy, vz, rho;
+    };
+    struct Acc {
+        double d;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
+        t.vx = COL(c, C_VX, p); t.vy = COL(c, C_VY, p); t.vz = COL(c, C_VZ, p);
+        t.rho = COL(c, C_RHO, p);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    {
+        stage_pos64(c, q, l, s);
+        s.vx[l] = COL(c, C_VX, q); s.vy[l] = COL(c, C_VY, q); s.vz[l] = COL(c, C_VZ, q);
+        s.rho[l] = COL(c, C_RHO, q);
+    }
+    static __device__ void init(Acc& a) { a.d = 0.0; }
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    {
+        const double r = sqrt(ss);
+        const double u = 1.0 - r * c.g.inv_h;
+        double d = dx * (t.vx - s.vx[jj]) + dy * (t.vy - s.vy[jj]);
+        d = d + dz * (t.vz - s.vz[jj]);
+        a.d += u * u * u * (d + 2.0 * c.ph.nu * (t.rho - s.rho[jj]));
+    }
+    static __device__ void reduce(Acc& a, int T)
+    {
+        for (int o = T; o < 64; o <<= 1) a.d += __shfl_xor(a.d, o);
+    }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
+    {
+        double v = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
+        COL(c, C_DRHO, p) = v + c.ph.kdW * a.d;
+    }
+    static __device__ unsigned pairs(const Acc&) { return 0; }
+};
+
+// Accepted-pair count (COUNT) or emission of (refidx p, refidx q) (EMIT).
+template <bool EMIT>
+struct Pairs64 {
+    using Lds = LdsPos64;
+    struct Tgt {
+        double x, y, z;
+    };
+    struct Acc {
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s) { stage_pos64(c, q, l, s); }
+    static __device__ void init(Acc& a) { a.n = 0; }
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int q,
+                                    double, double, double, double, Acc& a)
+    {
+        if (EMIT) {
+            const unsigned long long k = atomicAdd(c.pair_counter, 1ull);
+            if ((long long)k < c.pair_cap) {
+                c.pair_out[2 * k] = c.refidx[p];
+                c.pair_out[2 * k + 1] = c.refidx[q];
+            }
+        } else {
+            a.n += 1;
+        }
+    }
+    static __device__ void reduce(Acc&, int) {}
+    static __device__ void store(const PairCtx&, int, const Tgt&, const Acc&) {}
+    static __device__ unsigned pairs(const Acc& a) { return EMIT ? 0u : a.n; }
+};
+
+// Row r of the 3 (2D) or 9 (3D) stencil rows: particle range [j0, j1).
+template <int ROWS>
+__device__ inline bool row_range(const PairCtx& c, int cell, int row, int& j0, int& j1)
+{
+    const int dj = row % 3 - 1;
+    const int dk = ROWS == 3 ? 0 : row / 3 - 1;
+    const long long mid = (long long)cell + (long long)c.g.L1 * dj +
+                          (long long)c.g.L1 * c.g.L2 * dk;
+    long long lo = mid - 1, hi = mid + 1;
+    if (lo < 0) lo = 0;
+    if (hi > c.g.key_max - 1) hi = c.g.key_max - 1;
+    if (lo > hi) return false;
+    j0 = c.cell_start[lo];
+    j1 = c.cell_start[hi + 1];
+    return j0 < j1;
+}
+
+// Tiles come from a work queue (one atomic per tile); every wave leaves
+// the loop once the queue passes the tile count.
+__device__ inline int next_tile(const PairCtx& c)
+{
+    int t = 0;
+    if (__lane_id() == 0) t = atomicAdd(c.queue, 1);
+    return __shfl(t, 0);
+}
+
+template <int ROWS, bool FLAT, class Op>
+__device__ inline void tile64(const PairCtx& c, int tile, typename Op::Lds& lds)
+{
+    const int2 td = c.tiles[tile];
+    const int cell = td.x, first = td.y;
+    const int cend = c.cell_start[cell + 1];
+    const int lane = threadIdx.x;
+    const int T = c.T, S = 64 / T;
+    const int t = lane & (T - 1), s = lane / T;
+    const int p = first + t;
+    const bool valid = p < cend;
+    typename Op::Tgt tg;
+    Op::load(c, valid ? p : first, tg);
+    const bool act = valid && Op::active(c, p, tg);
+    if (!__any(act)) return;
+    typename Op::Acc acc;
+    Op::init(acc);
+    const double s_thr = c.g.s_thr;
+    for (int row = 0; row < ROWS; ++row) {
+        int j0, j1;
+        if (!row_range<ROWS>(c, cell, row, j0, j1)) continue;
+        for (int b = j0; b < j1; b += 64) {
+            const int nb = min(64, j1 - b);
+            if (lane < nb) Op::stage(c, b + lane, lane, lds);
+            __syncthreads();
+            for (int j = 0; j < nb; j += S) {
+                const int jj = j + s;
+                if (jj < nb) {
+                    const double dx = tg.x - lds.x[jj];
+                    const double dy = tg.y - lds.y[jj];
+                    const double dz = FLAT ? 0.0 : tg.z - lds.z[jj];
+                    double ss = dx * dx + dy * dy;
+                    if (!FLAT) ss = ss + dz * dz;
+                    const int q = b + jj;
+                    if (act && ss <= s_thr && q != p)
+                        Op::interact(c, tg, lds, jj, p, q, dx, dy, dz, ss, acc);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (c.pair_counter) count_pairs(c, Op::pairs(acc));
+    Op::reduce(acc, T);
+    if (act && s == 0) Op::store(c, p, tg, acc);
+}
+
+template <int ROWS, bool FLAT, class Op>
+__global__ __launch_bounds__(64) void k_pair64(PairCtx c)
+{
+    __shared__ typename Op::Lds lds;
+    const int ntiles = c.meta[META_NTILES];
+    for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
+        tile64<ROWS, FLAT, Op>(c, tile, lds);
+}
+
+// =============================================================== mixed ====
+// FP32 pair arithmetic on coordinates relative to the tile origin (the FP64
+// position of the tile's first target), converted while staging.  Per-lane
+// partial sums are plain FP32 within a 64-candidate chunk and folded into a
+// compensated (sum, error) pair after every chunk.
+
+struct LdsPos32 {
+    float x[64], y[64], z[64];
+};
+struct LdsForce32 {
+    float x[64], y[64], z[64], vx[64], vy[64], vz[64], pr[64], rho[64];
+};
+
+struct Density32 {
+    using Lds = LdsPos32;
+    struct Tgt {
+        float x, y, z;
+    };
+    struct Part {
+        float w;
+    };
+    struct Acc {
+        float s, e;
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
+                                 Lds& s)
+    {
+        s.x[l] = (float)(COL(c, C_X, q) - ox);
+        s.y[l] = (float)(COL(c, C_Y, q) - oy);
+        s.z[l] = (float)(COL(c, C_Z, q) - oz);
+    }
+    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; a.n = 0; }
+    static __device__ void zero(Part& q) { q.w = 0.0f; }
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int, int,
+                                    float, float, float, float ss, Part& q, Acc& a)
+    {
+        const float r = __builtin_sqrtf(ss);
+        const float x = r * (float)c.g.inv_h;
+        const float u = 1.0f - x;
+        const float u2 = u * u;
+        q.w = fmaf(u2 * u2, fmaf(4.0f, x, 1.0f), q.w);
+        a.n += 1;
+    }
+    static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.w); }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
+                                 bool write, int out_col)
+    {
+        double w = (double)a.s + (double)a.e;
+        for (int o = T; o < 64; o <<= 1) w += __shfl_xor(w, o);
+        if (!write) return;
+        double v = c.overwrite ? 0.0 : COL(c, out_col, p);
+        v = v + c.ph.kW * w;
+        if (c.self) v = v + c.ph.kW;
+        COL(c, out_col, p) = v;
+    }
+    static __device__ unsigned pairs(const Acc& a) { return a.n; }
+};
+
+struct Force32 {
+    using Lds = LdsForce32;
+    struct Tgt {
+        float x, y, z, vx, vy, vz, pr, rho;
+        bool fluid;
+    };
+    struct Part {
+        float ax, ay, az, lx, ly, lz;
+    };
+    struct Acc {
+        float sx, sy, sz, ex, ey, ez, lx, ly, lz, fx, fy, fz;
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+        t.vx = (float)COL(c, C_VX, p); t.vy = (float)COL(c, C_VY, p); t.vz = (float)COL(c, C_VZ, p);
+        t.pr = (float)c.prho2[p];
+        t.rho = (float)COL(c, C_RHO, p);
+        t.fluid = COL(c, C_TYPE, p) == 0.0;
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt& t) { return t.fluid; }
+    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
+                                 Lds& s)
+    {
+        s.x[l] = (float)(COL(c, C_X, q) - ox);
+        s.y[l] = (float)(COL(c, C_Y, q) - oy);
+        s.z[l] = (float)(COL(c, C_Z, q) - oz);
+        s.vx[l] = (float)COL(c, C_VX, q); s.vy[l] = (float)COL(c, C_VY, q);
+        s.vz[l] = (float)COL(c, C_VZ, q);
+        s.pr[l] = (float)c.prho2[q];
+        s.rho[l] = (float)COL(c, C_RHO, q);
+    }
+    static __device__ void init(Acc& a)
+    {
+        a.sx = a.sy = a.sz = a.ex = a.ey = a.ez = 0.0f;
+        a.lx = a.ly = a.lz = a.fx = a.fy = a.fz = 0.0f;
+        a.n = 0;
+    }
+    static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int, int,
+                                    float dx, float dy, float dz, float ss, Part& q, Acc& a)
+    {
+        a.n += 1;
+        const float r = __builtin_sqrtf(ss);
+        const float u = 1.0f - r * (float)c.g.inv_h;
+        const float u3 = u * u * u;
+        float A = t.pr + s.pr[jj];
+        const float dvx = t.vx - s.vx[jj], dvy = t.vy - s.vy[jj], dvz = t.vz - s.vz[jj];
+        if (c.ph.avk != 0.0) {
+            const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
+            if (vx < 0.0f) {
+                const float rhobar = 0.5f * (t.rho + s.rho[jj]);
+                const float den = rhobar * (ss + (float)c.ph.eps);
+                A = fmaf((float)c.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
+            }
+        }
+        const float f = u3 * A;
+        q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
+        if (c.ph.mu != 0.0) {
+            q.lx = fmaf(u3, dvx, q.lx); q.ly = fmaf(u3, dvy, q.ly); q.lz = fmaf(u3, dvz, q.lz);
+        }
+    }
+    static __device__ void fold(Acc& a, const Part& q)
+    {
+        two_sum_add(a.sx, a.ex, q.ax);
+        two_sum_add(a.sy, a.ey, q.ay);
+        two_sum_add(a.sz, a.ez, q.az);
+        two_sum_add(a.lx, a.fx, q.lx);
+        two_sum_add(a.ly, a.fy, q.ly);
+        two_sum_add(a.lz, a.fz, q.lz);
+    }
+    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
+                                 bool write, int)
+    {
+        double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
+               az = (double)a.sz + (double)a.ez;
+        double lx = (double)a.lx + (double)a.fx, ly = (double)a.ly + (double)a.fy,
+               lz = (double)a.lz + (double)a.fz;
+        for (int o = T; o < 64; o <<= 1) {
+            ax += __shfl_xor(ax, o); ay += __shfl_xor(ay, o); az += __shfl_xor(az, o);
+            lx += __shfl_xor(lx, o); ly += __shfl_xor(ly, o); lz += __shfl_xor(lz, o);
+        }
+        if (!write) return;
+        const double kp = -c.ph.kdW;
+        const double kl = 2.0 * c.ph.kdW * c.ph.mu / (c.ph.rho0 * c.ph.rho0);
+        double ox = c.overwrite ? 0.0 : COL(c, C_AX, p);
+        double oy = c.overwrite ? 0.0 : COL(c, C_AY, p);
+        double oz = c.overwrite ? 0.0 : COL(c, C_AZ, p);
+        ox = ox + kp * ax; oy = oy + kp * ay; oz = oz + kp * az;
+        if (c.ph.mu != 0.0) { ox = ox + kl * lx; oy = oy + kl * ly; oz = oz + kl * lz; }
+        COL(c, C_AX, p) = ox; COL(c, C_AY, p) = oy; COL(c, C_AZ, p) = oz;
+    }
+    static __device__ unsigned pairs(const Acc& a) { return a.n; }
+};
+
+template <bool EMIT>
+struct Pairs32 {
+    using Lds = LdsPos32;
+    struct Tgt {
+        float x, y, z;
+    };
+    struct Part {
+        int dummy;
+    };
+    struct Acc {
+        unsigned n;
+    };
+    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+    }
+    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
+                                 Lds& s)
+    {
+        s.x[l] = (float)(COL(c, C_X, q) - ox);
+        s.y[l] = (float)(COL(c, C_Y, q) - oy);
+        s.z[l] = (float)(COL(c, C_Z, q) - oz);
+    }
+    static __device__ void init(Acc& a) { a.n = 0; }
+    static __device__ void zero(Part&) {}
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int q,
+                                    float, float, float, float, Part&, Acc& a)
+    {
+        if (EMIT) {
+            const unsigned long long k = atomicAdd(c.pair_counter, 1ull);
+            if ((long long)k < c.pair_cap) {
+                c.pair_out[2 * k] = c.refidx[p];
+                c.pair_out[2 * k + 1] = c.refidx[q];
+            }
+        } else {
+            a.n += 1;
+        }
+    }
+    static __device__ void fold(Acc&, const Part&) {}
+    static __device__ void store(const PairCtx&, int, const Tgt&, const Acc&, int, bool, int) {}
+    static __device__ unsigned pairs(const Acc& a) { return EMIT ? 0u : a.n; }
+};
+
+template <int ROWS, bool FLAT, class Op>
+__device__ inline void tile32(const PairCtx& c, int tile, int out_col, typename Op::Lds& lds)
+{
+    const int2 td = c.tiles[tile];
+    const int cell = td.x, first = td.y;
+    const int cend = c.cell_start[cell + 1];
+    const int lane = threadIdx.x;
+    const int T = c.T, S = 64 / T;
+    const int t = lane & (T - 1), s = lane / T;
+    const int p = first + t;
+    const bool valid = p < cend;
+    const int pp = valid ? p : first;
+    const double ox = COL(c, C_X, first), oy = COL(c, C_Y, first), oz = COL(c, C_Z, first);
+    typename Op::Tgt tg;
+    Op::load(c, pp, ox, oy, oz, tg);
+    const bool act = valid && Op::active(c, p, tg);
+    if (!__any(act)) return;
+    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
+    const double s_thr = c.g.s_thr;
+    const float thr_lo = (float)(s_thr * (1.0 - BAND));
+    const float thr_hi = (float)(s_thr * (1.0 + BAND));
+    typename Op::Acc acc;
+    Op::init(acc);
+    for (int row = 0; row < ROWS; ++row) {
+        int j0, j1;
+        if (!row_range<ROWS>(c, cell, row, j0, j1)) continue;
+        for (int b = j0; b < j1; b += 64) {
+            const int nb = min(64, j1 - b);
+            if (lane < nb) Op::stage(c, b + lane, lane, ox, oy, oz, lds);
+            __syncthreads();
+            typename Op::Part part;
+            Op::zero(part);
+            for (int j = 0; j < nb; j += S) {
+                const int jj = j + s;
+                const bool in = jj < nb;
+                const int jr = in ? jj : 0;
+                const float dx = tg.x - lds.x[jr];
+                const float dy = tg.y - lds.y[jr];
+                const float dz = FLAT ? 0.0f : tg.z - lds.z[jr];
+                const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const int q = b + jr;
+                bool ok = ss <= thr_lo;
+                const bool band = in && act && ss > thr_lo && ss <= thr_hi;
+                if (__any(band)) {
+                    if (band) {
+                        // exact FP64 predicate of the reference
+                        const double e1 = px - COL(c, C_X, q), e2 = py - COL(c, C_Y, q);
+                        double es = e1 * e1 + e2 * e2;
+                        if (!FLAT) {
+                            const double e3 = pz - COL(c, C_Z, q);
+                            es = es + e3 * e3;
+                        }
+                        ok = es <= s_thr;
+                    }
+                }
+                if (in && act && ok && q != p)
+                    Op::interact(c, tg, lds, jr, p, q, dx, dy, dz, ss, part, acc);
+            }
+            Op::fold(acc, part);
+            __syncthreads();
+        }
+    }
+    if (c.pair_counter) count_pairs(c, Op::pairs(acc));
+    Op::store(c, p, tg, acc, T, act && s == 0, out_col);
+}
+
+template <int ROWS, bool FLAT, class Op>
+__global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
+{
+    __shared__ typename Op::Lds lds;
+    const int ntiles = c.meta[META_NTILES];
+    for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
+        tile32<ROWS, FLAT, Op>(c, tile, out_col, lds);
+}
+
+// ========================================================= diagnostics ====
+// Candidates tested per pass: for every particle, the particles of the 27
+// (9) cells it visits (src/core.jl:96-102).  out[0] over all targets,
+// out[1] over fluid targets (the force pass).
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_candidates(PairCtx c, const int* __restrict__ key,
+                                                    unsigned long long* out)
+{
+    const int n = c.meta[META_N];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long all = 0, fluid = 0;
+    if (p < n) {
+        const int cell = key[p];
+        for (int row = 0; row < ROWS; ++row) {
+            int j0, j1;
+            if (row_range<ROWS>(c, cell, row, j0, j1)) all += (unsigned long long)(j1 - j0);
+        }
+        if (COL(c, C_TYPE, p) == 0.0) fluid = all;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        all += __shfl_xor(all, o);
+        fluid += __shfl_xor(fluid, o);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(&out[0], all);
+        atomicAdd(&out[1], fluid);
+    }
+}
+
+void launch_candidates(const PairCtx& c, const int* key, unsigned long long* out, long long n,
+                       hipStream_t st)
+{
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (nb == 0) return;
+    if (c.g.dim == 2) k_candidates<3><<<nb, 256, 0, st>>>(c, key, out);
+    else k_candidates<9><<<nb, 256, 0, st>>>(c, key, out);
+}
+
+// ============================================================ dispatch ====
+
+template <int ROWS, bool FLAT>
+static void launch_rows(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
+{
+    if (precision == SPH_FP64) {
+        switch (op) {
+        case P_DENSITY: k_pair64<ROWS, FLAT, Density64<C_RHO>><<<grid, 64, 0, st>>>(c); break;
+        case P_DENSITY_RHO0: k_pair64<ROWS, FLAT, Density64<C_RHO0>><<<grid, 64, 0, st>>>(c); break;
+        case P_FORCE: k_pair64<ROWS, FLAT, Force64><<<grid, 64, 0, st>>>(c); break;
+        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, ForceRho064><<<grid, 64, 0, st>>>(c); break;
+        case P_CONTINUITY: k_pair64<ROWS, FLAT, Continuity64><<<grid, 64, 0, st>>>(c); break;
+        case P_COUNT: k_pair64<ROWS, FLAT, Pairs64<false>><<<grid, 64, 0, st>>>(c); break;
+        case P_EMIT: k_pair64<ROWS, FLAT, Pairs64<true>><<<grid, 64, 0, st>>>(c); break;
+        }
+    } else {
+        switch (op) {
+        case P_DENSITY: k_pair32<ROWS, FLAT, Density32><<<grid, 64, 0, st>>>(c, C_RHO); break;
+        case P_DENSITY_RHO0: k_pair32<ROWS, FLAT, Density32><<<grid, 64, 0, st>>>(c, C_RHO0); break;
+        case P_FORCE: k_pair32<ROWS, FLAT, Force32><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_COUNT: k_pair32<ROWS, FLAT, Pairs32<false>><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_EMIT: k_pair32<ROWS, FLAT, Pairs32<true>><<<grid, 64, 0, st>>>(c, 0); break;
+        // operators without a mixed form run in FP64
+        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, ForceRho064><<<grid, 64, 0, st>>>(c); break;
+        case P_CONTINUITY: k_pair64<ROWS, FLAT, Continuity64><<<grid, 64, 0, st>>>(c); break;
+        }
+    }
+}
+
+void launch_pair(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
+{
+    if (grid <= 0) return;
+    if (c.g.dim == 2) {
+        if (c.g.flat) launch_rows<3, true>(op, precision, c, grid, st);
+        else launch_rows<3, false>(op, precision, c, grid, st);
+    } else {
+        launch_rows<9, false>(op, precision, c, grid, st);
+    }
+}
+
+}  // namespace sph

@@ -1,0 +1,290 @@
+// Unary operators (apply_unary!, src/core.jl:138-142), global reductions
+// and the field transfers of ParticleField (src/structs.jl:118-125).
+//
+// All kernels run over the cell-sorted device order and read the particle
+// count from meta[META_N], so they can be enqueued without a host round
+// trip.  The arithmetic of every unary operator is the reference's FP64
+// expression, evaluated in the same order, so these are bit-exact.
+#include "sph_internal.h"
+
+namespace sph {
+
+#define UCOL(cols, cap, f, i) ((cols)[(long long)(f) * (cap) + (i)])
+
+// ------------------------------------------------------------ operators --
+__global__ __launch_bounds__(256) void k_unary(int op, const int* meta, double* cols,
+                                               long long cap, Phys ph, double* prho2)
+{
+    const int n = meta[META_N];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    switch (op) {
+    case SPH_OP_RESET_RHO:  // tests/test_collision_2d.jl:84-86
+        UCOL(cols, cap, C_RHO, i) = 0.0;
+        break;
+    case SPH_OP_RESET_A:  // tests/test_collision_2d.jl:80-82
+        UCOL(cols, cap, C_AX, i) = 0.0;
+        UCOL(cols, cap, C_AY, i) = 0.0;
+        UCOL(cols, cap, C_AZ, i) = 0.0;
+        break;
+    case SPH_OP_FIND_PRESSURE: {  // examples/collapse_dry.jl:126
+        const double rho = UCOL(cols, cap, C_RHO, i);
+        const double P = (ph.c * ph.c) * (rho - ph.rho0);
+        UCOL(cols, cap, C_P, i) = P;
+        if (prho2) prho2[i] = P / (rho * rho);
+        break;
+    }
+    case SPH_OP_FIND_PRESSURE_RHO0: {  // tests/test_collision_2d.jl:71-73
+        const double rho = UCOL(cols, cap, C_RHO, i);
+        UCOL(cols, cap, C_P, i) = (ph.c * ph.c) * (rho - UCOL(cols, cap, C_RHO0, i));
+        break;
+    }
+    case SPH_OP_FIND_PRESSURE_CONTINUITY: {  // examples/collapse_dry.jl:123-127
+        double rho = UCOL(cols, cap, C_RHO, i);
+        rho = rho + UCOL(cols, cap, C_DRHO, i) * ph.dt;
+        UCOL(cols, cap, C_RHO, i) = rho;
+        UCOL(cols, cap, C_DRHO, i) = 0.0;
+        UCOL(cols, cap, C_P, i) = (ph.c * ph.c) * (rho - ph.rho0);
+        break;
+    }
+    case SPH_OP_ACCELERATE: {  // examples/collapse_dry.jl:155-159
+        if (UCOL(cols, cap, C_TYPE, i) != 0.0) break;
+        const double k = ph.coeff * ph.dt;
+        for (int d = 0; d < 3; ++d)
+            UCOL(cols, cap, C_VX + d, i) =
+                UCOL(cols, cap, C_VX + d, i) + k * (UCOL(cols, cap, C_AX + d, i) + ph.g[d]);
+        break;
+    }
+    case SPH_OP_MOVE: {  // examples/collapse_dry.jl:148-153
+        if (UCOL(cols, cap, C_TYPE, i) != 0.0) break;
+        const double k = ph.coeff * ph.dt;
+        for (int d = 0; d < 3; ++d)
+            UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * UCOL(cols, cap, C_VX + d, i);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// P/rho^2 per particle, consumed by the force pass
+// (examples/collapse_dry.jl:138: p.P/p.rho^2 + q.P/q.rho^2).
+__global__ __launch_bounds__(256) void k_prho2(const int* meta, const double* cols, long long cap,
+                                               double* prho2)
+{
+    const int n = meta[META_N];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double rho = UCOL(cols, cap, C_RHO, i);
+    prho2[i] = UCOL(cols, cap, C_P, i) / (rho * rho);
+}
+
+// Fused kick and drift of the symplectic Euler step (fluid only):
+// v += dt (a + g); x += dt v  (examples/collapse_dry.jl:148-159 with a
+// full step instead of two half steps).
+__global__ __launch_bounds__(256) void k_kick_drift(const int* meta, double* cols, long long cap,
+                                                    Phys ph)
+{
+    const int n = meta[META_N];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (UCOL(cols, cap, C_TYPE, i) != 0.0) return;
+    const double k = ph.coeff * ph.dt;
+    for (int d = 0; d < 3; ++d) {
+        const double v = UCOL(cols, cap, C_VX + d, i) + k * (UCOL(cols, cap, C_AX + d, i) + ph.g[d]);
+        UCOL(cols, cap, C_VX + d, i) = v;
+        UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * v;
+    }
+}
+
+// ----------------------------------------------------------- reductions --
+// Deterministic two-level tree: per-block partial sums in a fixed order,
+// then one block sums the partials in a fixed order.
+
+__device__ inline double block_sum(double v)
+{
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+static constexpr int RED_ITEMS = 8;
+
+__device__ inline double energy_of(int what, const double* cols, long long cap, const Phys& ph,
+                                   int i)
+{
+    const double v0 = UCOL(cols, cap, C_VX, i), v1 = UCOL(cols, cap, C_VY, i),
+                 v2 = UCOL(cols, cap, C_VZ, i);
+    double vv = v0 * v0 + v1 * v1;
+    vv = vv + v2 * v2;
+    const double rho = UCOL(cols, cap, C_RHO, i);
+    if (what == SPH_REDUCE_ENERGY) {  // examples/collapse_dry.jl:166-171
+        const double x0 = UCOL(cols, cap, C_X, i), x1 = UCOL(cols, cap, C_Y, i),
+                     x2 = UCOL(cols, cap, C_Z, i);
+        double gx = ph.g[0] * x0 + ph.g[1] * x1;
+        gx = gx + ph.g[2] * x2;
+        const double kinetic = 0.5 * ph.m * vv;
+        const double potential = -ph.m * gx;
+        const double internal =
+            ph.m * (ph.c * ph.c) * (log(fabs(rho / ph.rho0)) + ph.rho0 / rho - 1.0);
+        return kinetic + potential + internal;
+    }
+    // tests/test_collision_2d.jl:96-100
+    const double d = rho - UCOL(cols, cap, C_RHO0, i);
+    return 0.5 * ph.m * vv + 0.5 * ph.m * (ph.c * ph.c) * (d * d) / (ph.rho0 * ph.rho0);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_part(int what, const int* meta,
+                                                     const double* cols, long long cap, Phys ph,
+                                                     double* part)
+{
+    const int n = meta[META_N];
+    const int base = blockIdx.x * 256 * RED_ITEMS;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int it = 0; it < RED_ITEMS; ++it) {
+        const int i = base + it * 256 + threadIdx.x;
+        if (i >= n) break;
+        if (what == SPH_REDUCE_MOMENTUM) {
+            a0 += ph.m * UCOL(cols, cap, C_VX, i);
+            a1 += ph.m * UCOL(cols, cap, C_VY, i);
+            a2 += ph.m * UCOL(cols, cap, C_VZ, i);
+        } else {
+            a0 += energy_of(what, cols, cap, ph, i);
+        }
+    }
+    const double s0 = block_sum(a0);
+    const double s1 = what == SPH_REDUCE_MOMENTUM ? block_sum(a1) : 0.0;
+    const double s2 = what == SPH_REDUCE_MOMENTUM ? block_sum(a2) : 0.0;
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = s0;
+        part[3 * blockIdx.x + 1] = s1;
+        part[3 * blockIdx.x + 2] = s2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nb, double* out)
+{
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int d = 0; d < 3; ++d) a[d] += part[3 * b + d];
+    for (int d = 0; d < 3; ++d) {
+        const double s = block_sum(a[d]);
+        if (threadIdx.x == 0) out[d] = s;
+    }
+}
+
+// ------------------------------------------------------------ field I/O --
+// Host order is the reference order: host slot refidx[k] <-> device slot k.
+__global__ __launch_bounds__(256) void k_field_out(const int* meta, const int* __restrict__ ref,
+                                                   const double* cols, long long cap, int col0,
+                                                   int ncomp, double* out)
+{
+    const int n = meta[META_N];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const long long r = ref[k];
+    for (int d = 0; d < ncomp; ++d) out[ncomp * r + d] = UCOL(cols, cap, col0 + d, k);
+}
+
+__global__ __launch_bounds__(256) void k_field_in(const int* meta, const int* __restrict__ ref,
+                                                  double* cols, long long cap, int col0, int ncomp,
+                                                  const double* in)
+{
+    const int n = meta[META_N];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const long long r = ref[k];
+    for (int d = 0; d < ncomp; ++d) UCOL(cols, cap, col0 + d, k) = in[ncomp * r + d];
+}
+
+// Append m particles at device slots [n0, n0+m) with reference indices
+// n0.. ; `in` holds x[m][3], v[m][3], rho[m], P[m], type[m] back to back.
+__global__ __launch_bounds__(256) void k_append(int n0, int m, const double* in, int has_v,
+                                                int has_rho, int has_P, int has_type,
+                                                double* cols, int* icols, long long cap)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const int k = n0 + i;
+    const double* x = in;
+    const double* v = in + 3ll * m;
+    const double* rho = in + 6ll * m;
+    const double* P = in + 7ll * m;
+    const double* type = in + 8ll * m;
+    for (int f = 0; f < NCOL; ++f) UCOL(cols, cap, f, k) = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        UCOL(cols, cap, C_X + d, k) = x[3ll * i + d];
+        if (has_v) UCOL(cols, cap, C_VX + d, k) = v[3ll * i + d];
+    }
+    if (has_rho) UCOL(cols, cap, C_RHO, k) = rho[i];
+    if (has_P) UCOL(cols, cap, C_P, k) = P[i];
+    if (has_type) UCOL(cols, cap, C_TYPE, k) = type[i];
+    icols[I_REF * cap + k] = k;
+    icols[I_KEY * cap + k] = 0;
+}
+
+__global__ void k_set_meta(int* meta, int idx, int v) { meta[idx] = v; }
+
+// ------------------------------------------------------------ launchers --
+static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+void enqueue_unary(sph_system* s, int op, const Phys& ph, double* prho2)
+{
+    if (s->n <= 0) return;
+    k_unary<<<nblk(s->n), 256, 0, s->stream>>>(op, s->meta, s->cols[s->cur], s->cap, ph, prho2);
+}
+
+void enqueue_prho2(sph_system* s)
+{
+    if (s->n <= 0) return;
+    k_prho2<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, s->prho2);
+}
+
+void enqueue_kick_drift(sph_system* s, const Phys& ph)
+{
+    if (s->n <= 0) return;
+    k_kick_drift<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, ph);
+}
+
+long long reduce_partials_needed(long long n) { return (n + 256 * RED_ITEMS - 1) / (256 * RED_ITEMS) + 1; }
+
+void enqueue_reduce(sph_system* s, int what, const Phys& ph, double* out_dev)
+{
+    const int nb = (int)((s->n + 256 * RED_ITEMS - 1) / (256 * RED_ITEMS));
+    if (nb > 0)
+        k_reduce_part<<<nb, 256, 0, s->stream>>>(what, s->meta, s->cols[s->cur], s->cap, ph,
+                                                 s->red_part);
+    k_reduce_final<<<1, 256, 0, s->stream>>>(s->red_part, nb, out_dev);
+}
+
+void enqueue_field_out(sph_system* s, int col0, int ncomp, double* out_dev)
+{
+    if (s->n <= 0) return;
+    k_field_out<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->icols[s->cur] + I_REF * s->cap,
+                                                  s->cols[s->cur], s->cap, col0, ncomp, out_dev);
+}
+
+void enqueue_field_in(sph_system* s, int col0, int ncomp, const double* in_dev)
+{
+    if (s->n <= 0) return;
+    k_field_in<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->icols[s->cur] + I_REF * s->cap,
+                                                 s->cols[s->cur], s->cap, col0, ncomp, in_dev);
+}
+
+void enqueue_append(sph_system* s, int n0, int m, const double* in_dev, int has_v, int has_rho,
+                    int has_P, int has_type)
+{
+    if (m <= 0) return;
+    k_append<<<nblk(m), 256, 0, s->stream>>>(n0, m, in_dev, has_v, has_rho, has_P, has_type,
+                                            s->cols[s->cur], s->icols[s->cur], s->cap);
+    k_set_meta<<<1, 1, 0, s->stream>>>(s->meta, META_N, n0 + m);
+}
+
+}  // namespace sph

@@ -1,0 +1,345 @@
+"""smoothedparticles_hip -- host-side mirror of SmoothedParticles.jl's
+hot-path API over the gfx950 library libsph_hip (C ABI, include/sph.h).
+
+Reference call surface and its counterpart here:
+
+    ParticleSystem(T, domain, h)       ParticleSystem(domain, h, precision)
+    generate_particles!/push!          sys.add_particles(x, v, rho, P, type)
+    ParticleField(sys, :rho)           ParticleField(sys, "rho")
+    create_cell_list!(sys)             create_cell_list(sys)
+    apply!(sys, op!; self, params...)  apply(sys, op, self=..., **params)
+    apply_binary!(sys, op!)            apply_binary(sys, op, **params)
+    apply_unary!(sys, op!)             apply_unary(sys, op, **params)
+    sum(p -> energy(p), sys.particles) energy(sys, **params)
+
+Operators cannot be arbitrary closures across the C ABI; the built-in ones
+are registered by the reference name they replace (``OPERATORS``).
+Everything runs through the HIP library; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+from ._native import SPHError, lib, check
+from .kernels import wendland2, rDwendland2, wendland3, rDwendland3  # noqa: F401
+
+__all__ = [
+    "Box", "Rectangle", "ParticleSystem", "ParticleField", "create_cell_list",
+    "apply", "apply_binary", "apply_unary", "energy", "momentum", "neighbour_pairs",
+    "step", "OPERATORS", "SPHError", "find_rho", "find_rho0", "internal_force",
+    "internal_force_rho0", "balance_of_mass", "reset_rho", "reset_a", "find_pressure",
+    "find_pressure_rho0", "find_pressure_continuity", "accelerate", "move",
+    "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL",
+]
+
+FLUID = 0.0
+WALL = 1.0
+
+_dptr = ctypes.POINTER(ctypes.c_double)
+
+
+def _as_ptr(a):
+    return a.ctypes.data_as(_dptr)
+
+
+class Box:
+    """Box(x1_min, x2_min, x3_min, x1_max, x2_max, x3_max), src/geometry.jl:15-22."""
+
+    def __init__(self, x1_min, x2_min, x3_min, x1_max, x2_max, x3_max):
+        self.lo = np.array([x1_min, x2_min, x3_min], dtype=np.float64)
+        self.hi = np.array([x1_max, x2_max, x3_max], dtype=np.float64)
+
+
+def Rectangle(x1_min, x2_min, x1_max, x2_max):
+    """Rectangle = Box with x3 in [0, 0] (src/geometry.jl:41-43)."""
+    return Box(x1_min, x2_min, 0.0, x1_max, x2_max, 0.0)
+
+
+class Operator:
+    """A registered operator: its library id and whether it is binary."""
+
+    def __init__(self, name, op_id, binary):
+        self.name = name
+        self.id = op_id
+        self.binary = binary
+
+    def __repr__(self):
+        return f"<operator {self.name}>"
+
+
+find_rho = Operator("find_rho!", N.SPH_OP_DENSITY_SUM, True)
+find_rho0 = Operator("find_rho0!", N.SPH_OP_DENSITY_SUM_RHO0, True)
+internal_force = Operator("internal_force!", N.SPH_OP_INTERNAL_FORCE, True)
+internal_force_rho0 = Operator("internal_force! (rho0)", N.SPH_OP_PRESSURE_FORCE_RHO0, True)
+balance_of_mass = Operator("balance_of_mass!", N.SPH_OP_BALANCE_OF_MASS, True)
+reset_rho = Operator("reset_rho!", N.SPH_OP_RESET_RHO, False)
+reset_a = Operator("reset_a!", N.SPH_OP_RESET_A, False)
+find_pressure = Operator("find_pressure!", N.SPH_OP_FIND_PRESSURE, False)
+find_pressure_rho0 = Operator("find_pressure! (rho0 field)", N.SPH_OP_FIND_PRESSURE_RHO0, False)
+find_pressure_continuity = Operator(
+    "find_pressure! (continuity)", N.SPH_OP_FIND_PRESSURE_CONTINUITY, False)
+accelerate = Operator("accelerate!", N.SPH_OP_ACCELERATE, False)
+move = Operator("move!", N.SPH_OP_MOVE, False)
+
+# Reference operator name -> registered operator.
+OPERATORS = {
+    "find_rho!": find_rho,                       # tests/test_collision_2d.jl:63-65
+    "find_rho0!": find_rho0,                     # tests/test_collision_2d.jl:67-69
+    "internal_force!": internal_force,           # examples/collapse_dry.jl:135-141
+    "internal_force_rho0!": internal_force_rho0,  # tests/test_collision_2d.jl:75-78
+    "balance_of_mass!": balance_of_mass,         # examples/collapse_dry.jl:112-115
+    "reset_rho!": reset_rho,                     # tests/test_collision_2d.jl:84-86
+    "reset_a!": reset_a,                         # tests/test_collision_2d.jl:80-82
+    "find_pressure!": find_pressure,             # examples/collapse_dry.jl:126
+    "find_pressure_rho0!": find_pressure_rho0,   # tests/test_collision_2d.jl:71-73
+    "find_pressure_continuity!": find_pressure_continuity,  # examples/collapse_dry.jl:123-127
+    "accelerate!": accelerate,                   # examples/collapse_dry.jl:155-159
+    "move!": move,                               # examples/collapse_dry.jl:148-153
+}
+
+_PARAM_NAMES = ("m", "rho0", "c", "alpha", "hs", "mu", "nu", "g", "dt", "coeff")
+
+
+def make_params(**kw):
+    """Build an sph_params struct from keyword parameters (missing = 0,
+    coeff defaults to 1)."""
+    unknown = set(kw) - set(_PARAM_NAMES)
+    if unknown:
+        raise TypeError(f"unknown parameters: {sorted(unknown)}")
+    p = N.sph_params()
+    for k in ("m", "rho0", "c", "alpha", "hs", "mu", "nu", "dt"):
+        setattr(p, k, float(kw.get(k, 0.0)))
+    p.coeff = float(kw.get("coeff", 1.0))
+    g = kw.get("g", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.g[i] = float(g[i])
+    return p
+
+
+class ParticleSystem:
+    """ParticleSystem(domain, h, precision) -- src/structs.jl:43-92.
+
+    ``domain`` is a Box (only its bounding box matters, as in the
+    reference); ``h`` is the neighbour radius and cell size; ``precision``
+    is "fp64" (plain FP64, the drop-in) or "mixed" (FP32 pair arithmetic on
+    relative coordinates with compensated sums, FP64 state).
+    """
+
+    def __init__(self, domain, h, precision="fp64"):
+        if not (h > 0.0):
+            raise AssertionError("invalid ParticleSystem declaration! (h must be a positive float)")
+        mode = {"fp64": N.SPH_FP64, "mixed": N.SPH_MIXED}[precision]
+        self.h = float(h)
+        self.domain = domain
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        check(lib().sph_create(ctypes.byref(self._h), self.h, _as_ptr(domain.lo),
+                               _as_ptr(domain.hi), mode))
+
+    def close(self):
+        if self._h:
+            lib().sph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        n = ctypes.c_int64()
+        check(lib().sph_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    @property
+    def n(self):
+        return len(self)
+
+    def add_particles(self, x, v=None, rho=None, P=None, type=None):
+        """push! n particles (generate_particles!, src/grids.jl:253-258)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+
+        def opt(a, shape):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
+            return a
+
+        v = opt(v, (n, 3))
+        rho = opt(rho, (n,))
+        P = opt(P, (n,))
+        t = opt(type, (n,))
+        keep = [x, v, rho, P, t]  # noqa: F841 (lifetime during the call)
+        check(lib().sph_append(self._h, n, _as_ptr(x),
+                               _as_ptr(v) if v is not None else None,
+                               _as_ptr(rho) if rho is not None else None,
+                               _as_ptr(P) if P is not None else None,
+                               _as_ptr(t) if t is not None else None))
+
+    def get(self, name):
+        n = len(self)
+        shape = (n, 3) if name in N.VECTOR_FIELDS else (n,)
+        out = np.empty(shape, dtype=np.float64)
+        check(lib().sph_get_field(self._h, N.FIELDS[name], _as_ptr(out), n))
+        return out
+
+    def set(self, name, values):
+        n = len(self)
+        shape = (n, 3) if name in N.VECTOR_FIELDS else (n,)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), shape))
+        check(lib().sph_set_field(self._h, N.FIELDS[name], _as_ptr(a), n))
+
+    def key_geometry(self):
+        """(key_phase, key_lim, key_max, key_diff) as the reference computes them."""
+        ph = (ctypes.c_int64 * 3)()
+        lim = (ctypes.c_int64 * 3)()
+        km = ctypes.c_int64()
+        diff = (ctypes.c_int64 * 27)()
+        nd = ctypes.c_int()
+        check(lib().sph_key_geometry(self._h, ph, lim, ctypes.byref(km), diff, ctypes.byref(nd)))
+        return tuple(ph), tuple(lim), km.value, list(diff)[: nd.value]
+
+    def set_tile(self, T):
+        check(lib().sph_set_tile(self._h, int(T)))
+
+    def autotune(self, **params):
+        best = ctypes.c_int()
+        p = make_params(**params)
+        check(lib().sph_autotune(self._h, ctypes.byref(p), ctypes.byref(best)))
+        return best.value
+
+    def step_pairs(self):
+        """(density-pass pairs, force-pass pairs) of the last step() call."""
+        d = ctypes.c_int64()
+        f = ctypes.c_int64()
+        check(lib().sph_step_pairs(self._h, ctypes.byref(d), ctypes.byref(f)))
+        return d.value, f.value
+
+    def candidates(self):
+        """(candidates over all targets, over fluid targets) of one pass."""
+        a = ctypes.c_int64()
+        f = ctypes.c_int64()
+        check(lib().sph_candidates(self._h, ctypes.byref(a), ctypes.byref(f)))
+        return a.value, f.value
+
+    def profile(self, enable=True):
+        check(lib().sph_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib().sph_profile_read(self._h, kernel, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+class ParticleField:
+    """ParticleField(sys, name) -- src/structs.jl:118-125: the n-th element
+    is field `name` of the n-th particle (reference order)."""
+
+    def __init__(self, sys, name):
+        if name not in N.FIELDS:
+            raise KeyError(f"Variable {name} does not exist!")
+        self.sys = sys
+        self.name = name
+
+    def __len__(self):
+        return len(self.sys)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.sys.get(self.name)
+        return a if dtype is None else a.astype(dtype)
+
+    def __getitem__(self, i):
+        return self.sys.get(self.name)[i]
+
+    def __setitem__(self, i, val):
+        a = self.sys.get(self.name)
+        a[i] = val
+        self.sys.set(self.name, a)
+
+
+def create_cell_list(sys):
+    """create_cell_list!(sys) -- src/core.jl:51-90.  Returns the new count."""
+    n = ctypes.c_int64()
+    check(lib().sph_build_cells(sys._h, ctypes.byref(n)))
+    return n.value
+
+
+def _resolve(op):
+    if isinstance(op, Operator):
+        return op
+    if isinstance(op, str) and op in OPERATORS:
+        return OPERATORS[op]
+    raise NotImplementedError(
+        f"{op!r} is not a registered operator; only the built-in operators "
+        f"{sorted(OPERATORS)} run on the device")
+
+
+def apply_binary(sys, op, **params):
+    """apply_binary!(sys, op) -- src/core.jl:125-129."""
+    op = _resolve(op)
+    if not op.binary:
+        raise TypeError(f"{op.name} is unary")
+    p = make_params(**params)
+    check(lib().sph_apply_binary(sys._h, op.id, ctypes.byref(p), 0))
+
+
+def apply_unary(sys, op, **params):
+    """apply_unary!(sys, op) -- src/core.jl:138-142."""
+    op = _resolve(op)
+    if op.binary:
+        raise TypeError(f"{op.name} is binary")
+    p = make_params(**params)
+    check(lib().sph_apply_unary(sys._h, op.id, ctypes.byref(p)))
+
+
+def apply(sys, op, self=False, **params):
+    """apply!(sys, op; self=false, parameters...) -- src/core.jl:151-161."""
+    op = _resolve(op)
+    p = make_params(**params)
+    check(lib().sph_apply(sys._h, op.id, ctypes.byref(p), 1 if self else 0))
+
+
+def _reduce(sys, what, nout, **params):
+    p = make_params(**params)
+    out = (ctypes.c_double * 3)()
+    check(lib().sph_reduce(sys._h, what, ctypes.byref(p), out))
+    return list(out)[:nout]
+
+
+def energy(sys, kind="wcsph", **params):
+    """Total energy: kind "wcsph" = examples/collapse_dry.jl:166-171,
+    kind "collision" = tests/test_collision_2d.jl:96-100."""
+    what = {"wcsph": N.SPH_REDUCE_ENERGY, "collision": N.SPH_REDUCE_ENERGY_COLLISION}[kind]
+    return _reduce(sys, what, 1, **params)[0]
+
+
+def momentum(sys, **params):
+    """Sum of m v over all particles."""
+    return np.array(_reduce(sys, N.SPH_REDUCE_MOMENTUM, 3, **params))
+
+
+def neighbour_pairs(sys):
+    """All accepted ordered pairs (p, q) of the current cell list as
+    reference indices, shape (npairs, 2), unspecified order."""
+    n = ctypes.c_int64()
+    check(lib().sph_neighbour_pairs(sys._h, None, 0, ctypes.byref(n)))
+    total = n.value
+    out = np.empty((max(total, 1), 2), dtype=np.int64)
+    check(lib().sph_neighbour_pairs(
+        sys._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), total, ctypes.byref(n)))
+    return out[:total]
+
+
+def step(sys, nsteps, scheme="wcsph_se", **params):
+    """nsteps fused device-resident steps (density sum, EOS, internal force,
+    symplectic Euler kick and drift).  Returns the accepted ordered pairs
+    summed over the density passes."""
+    p = make_params(**params)
+    pairs = ctypes.c_int64()
+    check(lib().sph_step(sys._h, N.SPH_SCHEME_WCSPH_SE, ctypes.byref(p), int(nsteps),
+                         ctypes.byref(pairs)))
+    return pairs.value

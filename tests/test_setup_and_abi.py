@@ -1,0 +1,109 @@
+"""Host-side checks that need no GPU: the lattice generator against an
+independent numpy restatement of covering / BoundaryLayer
+(src/grids.jl:52-66,128-144, src/geometry.jl:198-234), the C-ABI symbol
+table of libsph_hip, and the operator registry of the Python mirror."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import smoothedparticles_hip as sp
+from smoothedparticles_hip import _native as N
+from smoothedparticles_hip import states
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def np_covering_box(dim, dr, lo, hi):
+    r = [range(math.floor(lo[d] / dr), math.ceil(hi[d] / dr) + 1) for d in range(dim)]
+    if dim == 2:
+        r.append(range(0, 1))
+    I, J, K = np.meshgrid(np.array(r[0], float), np.array(r[1], float), np.array(r[2], float),
+                          indexing="ij")
+    x = np.stack([I * dr, J * dr, (K * dr if dim == 3 else np.zeros_like(K))], -1).reshape(-1, 3)
+    inside = np.all((x >= np.asarray(lo)) & (x <= np.asarray(hi)), axis=1)
+    return x[inside]
+
+
+def np_walls(dim, dr, tlo, thi, w, lid_axis, lid):
+    tlo, thi = np.asarray(tlo, float), np.asarray(thi, float)
+    blo, bhi = tlo.copy(), thi.copy()
+    blo[:dim] -= w
+    bhi[:dim] += w
+    ball = np_covering_box(dim, dr, -np.full(3, w) * [1, 1, 1 if dim == 3 else 0],
+                           np.full(3, w) * [1, 1, 1 if dim == 3 else 0])
+    ball = ball[((ball[:, 0] - 0.0) ** 2 + (ball[:, 1] - 0.0) ** 2) + (ball[:, 2] - 0.0) ** 2 <= w * w]
+    cand = np_covering_box(dim, dr, blo, bhi)
+    cand = cand[cand[:, lid_axis] < lid]
+    in_tank = np.all((cand >= tlo) & (cand <= thi), axis=1)
+    cand = cand[~in_tank]
+    hit = np.zeros(len(cand), bool)
+    for dx in ball:
+        y = cand + dx
+        hit |= np.all((y >= tlo) & (y <= thi), axis=1)
+    return cand[hit]
+
+
+@pytest.mark.parametrize("dim,dr", [(2, 0.02), (3, 0.1)])
+def test_lattice_generator_matches_numpy(dim, dr):
+    if dim == 2:
+        tank, fluid, vert = ((0, 0, 0), (4, 3, 0)), ((0, 0, 0), (1, 2, 0)), 1
+    else:
+        tank, fluid, vert = ((0, 0, 0), (4, 1, 3)), ((0, 0, 0), (1, 1, 2)), 2
+    xf = states.lattice_box(dim, dr, fluid[0], fluid[1])
+    assert np.array_equal(xf, np_covering_box(dim, dr, fluid[0], fluid[1]))
+    xw = states.lattice_walls(dim, dr, tank[0], tank[1], 3.5 * dr, vert, tank[1][vert])
+    assert np.array_equal(xw, np_walls(dim, dr, tank[0], tank[1], 3.5 * dr, vert, tank[1][vert]))
+
+
+def test_dambreak_2d_counts():
+    st = states.dam_break(2, 0.02)
+    assert st.n_fluid == 51 * 101
+    assert st.n_wall == 1363
+    # FP64 rounding of x + dx leaves the right wall with two layers
+    xw = st.x[st.n_fluid:]
+    assert sorted(set(np.round(xw[xw[:, 0] > 4.0, 0], 9))) == [4.02, 4.04]
+    jit = st.x[: st.n_fluid, :2] - np_covering_box(2, 0.02, (0, 0, 0), (1, 2, 0))[:, :2]
+    assert np.abs(jit).max() <= 1e-6 * 0.02 and np.abs(jit).max() > 0
+
+
+def header_functions():
+    names = []
+    for h in ("sph.h", "sph_setup.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(sph_[a-z_0-9]+)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+def test_abi_exports_every_declared_symbol():
+    names = header_functions()
+    assert "sph_create" in names and "sph_lattice_box" in names
+    lib = ctypes.CDLL(N.LIB_PATH)  # loading needs no GPU
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(N.SIGNATURES)
+
+
+def test_params_struct_layout():
+    assert ctypes.sizeof(N.sph_params) == 12 * 8
+    p = sp.make_params(m=2.0, g=(0, -9.8, 0), dt=1e-4)
+    assert p.m == 2.0 and p.g[1] == -9.8 and p.coeff == 1.0
+    with pytest.raises(TypeError):
+        sp.make_params(bogus=1.0)
+
+
+def test_operator_registry():
+    assert sp.OPERATORS["find_rho!"] is sp.find_rho and sp.find_rho.binary
+    assert not sp.OPERATORS["move!"].binary
+    with pytest.raises(NotImplementedError):
+        sp._resolve(lambda p, q, r: None)
+
+
+def test_host_kernels_match_reference_constants():
+    h = 0.3
+    assert sp.wendland2(h, 0.0) == pytest.approx(7 / math.pi / h ** 2, rel=1e-15)
+    assert sp.wendland3(h, 2 * h) == 0.0
+    assert sp.rDwendland3(h, 0.0) == pytest.approx(-210 / math.pi / h ** 5, rel=1e-15)
