@@ -36,6 +36,12 @@ FORCE_BYTES_PER_PARTICLE = {"mixed": 48.0, "fp64": 92.0}
 FLOPS_PER_CANDIDATE = 9.0
 FLOPS_PER_FORCE_PAIR = 45.0
 
+# Density summation with the per-particle reference density rho0_p set by
+# one find_rho0! pass at t = 0 (tests/test_collision_2d.jl:123-126): the EOS
+# is P = c^2 (rho - rho0_p), so kernel truncation at walls and the free
+# surface does not turn into pressure.
+SCHEME = "wcsph_se_rho0"
+
 WORKLOADS = {
     "dambreak3d_4M": dict(dim=3, dr=0.0079, R_over_dr=4.8,
                           desc="3D WCSPH dam break, dr=0.0079, R=4.8dr"),
@@ -56,13 +62,13 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
     p = st.params()
     g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, precision)
     g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
-    sp.create_cell_list(g)
+    sp.init_reference_density(g, **p)
     T = g.autotune(**p) if autotune else None
     if warmup:
-        sp.step(g, warmup, **p)
+        sp.step(g, warmup, scheme=SCHEME, **p)
     g.profile(True)
     t0 = time.perf_counter()
-    pairs = sp.step(g, steps, **p)  # returns after a stream sync
+    pairs = sp.step(g, steps, scheme=SCHEME, **p)  # returns after a stream sync
     t1 = time.perf_counter()
     dens_pairs, force_pairs = g.step_pairs()
     force_ms, force_n = g.profile_read(sp._native.SPH_KERNEL_FORCE)
@@ -82,14 +88,51 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
     )
 
 
+def accuracy(st, checkpoints=(100, 1000)):
+    """Mixed against plain FP64 on the device, same initial state: max
+    relative density error, velocity error (relative to max |v|) and
+    relative total-energy error at each checkpoint."""
+    import numpy as np
+
+    import smoothedparticles_hip as sp
+
+    p = st.params()
+    sysm = {}
+    for prec in ("fp64", "mixed"):
+        g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, prec)
+        g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
+        sp.init_reference_density(g, **p)
+        sysm[prec] = g
+    out, done = {}, 0
+    for c in checkpoints:
+        for g in sysm.values():
+            sp.step(g, c - done, scheme=SCHEME, **p)
+        done = c
+        a, b = sysm["mixed"], sysm["fp64"]
+        if len(a) != len(b):
+            raise RuntimeError(f"particle counts differ: mixed {len(a)}, fp64 {len(b)}")
+        rho_a, rho_b = a.get("rho"), b.get("rho")
+        v_a, v_b = a.get("v"), b.get("v")
+        e_a, e_b = sp.energy(a, "wcsph_rho0", **p), sp.energy(b, "wcsph_rho0", **p)
+        out[str(c)] = dict(particles=len(a), rho_rel=float(np.max(np.abs(rho_a / rho_b - 1))),
+                           v_rel=float(np.max(np.abs(v_a - v_b)) / np.max(np.abs(v_b))),
+                           energy_rel=float(abs(e_a / e_b - 1)))
+        log(f"step {c}: {out[str(c)]}")
+    for g in sysm.values():
+        g.close()
+    return out
+
+
 def cpu_baseline(st):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     o = O.OracleSystem(st.R, st.bbox_lo, st.bbox_hi)
     o.load(st.x, st.v, st.rho, st.P, st.type)
+    o.create_cell_list()
+    o.density_sum_rho0(True, **st.params())
     t0 = time.perf_counter()
-    pairs = o.step_wcsph(1, **st.params())
+    pairs = o.step_wcsph(1, use_rho0=True, **st.params())
     el = time.perf_counter() - t0
     return dict(value=2.0 * pairs / el, unit="pair-interactions/s", cores=O.num_threads(),
                 kind="port",
@@ -107,6 +150,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="also report mixed-vs-FP64 errors at steps 100 and 1000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -180,6 +225,8 @@ def main():
         out["fp64_value"] = other["value"]
         out["mixed_over_fp64"] = res["value"] / other["value"]
         out["fp64_ms_per_step"] = 1e3 * other["elapsed"] / args.steps
+    if args.accuracy:
+        out["accuracy_vs_fp64"] = accuracy(st)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(st)

@@ -89,10 +89,16 @@ typedef struct sph_system sph_system;
 #define SPH_REDUCE_ENERGY 0           /* sum of energy(p), examples/collapse_dry.jl:166-171 (1 value) */
 #define SPH_REDUCE_ENERGY_COLLISION 1 /* tests/test_collision_2d.jl:96-100 (1 value) */
 #define SPH_REDUCE_MOMENTUM 2         /* sum of m v (3 values) */
+#define SPH_REDUCE_ENERGY_RHO0 3      /* energy with the per-particle reference density:
+                                         internal m c^2 (ln(rho/rho0_p) + rho0_p/rho - 1),
+                                         the energy of P = c^2 (rho - rho0_p) (1 value) */
 
 /* Fused step schemes (sph_step) */
-#define SPH_SCHEME_WCSPH_SE 0  /* build cells; rho = sum m W (self); EOS; a = internal
-                                  force; v += dt (a + g); x += dt v  (fluid only) */
+#define SPH_SCHEME_WCSPH_SE 0       /* build cells; rho = sum m W (self); P = c^2 (rho - rho0);
+                                       a = internal force; v += dt (a + g); x += dt v (fluid) */
+#define SPH_SCHEME_WCSPH_SE_RHO0 1  /* the same with P = c^2 (rho - rho0_p), rho0_p the
+                                       per-particle field set by find_rho0! at t = 0
+                                       (tests/test_collision_2d.jl:67-73,123-126) */
 
 typedef struct sph_params {
     double m;      /* particle mass */

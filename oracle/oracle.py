@@ -64,7 +64,7 @@ def lib():
             "orc_move": (None, [ctypes.c_void_p, _pp]),
             "orc_energy": (ctypes.c_double, [ctypes.c_void_p, _pp, ctypes.c_int]),
             "orc_momentum": (None, [ctypes.c_void_p, _pp, _pd]),
-            "orc_step_wcsph": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int]),
+            "orc_step_wcsph": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int, ctypes.c_int]),
             "orc_num_threads": (ctypes.c_int, []),
             "orc_wendland2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "orc_rDwendland2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
@@ -214,7 +214,8 @@ class OracleSystem:
 
     def energy(self, kind="wcsph", **kw):
         p = params(**kw)
-        return lib().orc_energy(self._s, ctypes.byref(p), 0 if kind == "wcsph" else 1)
+        k = {"wcsph": 0, "collision": 1, "wcsph_rho0": 2}[kind]
+        return lib().orc_energy(self._s, ctypes.byref(p), k)
 
     def momentum(self, **kw):
         p = params(**kw)
@@ -222,9 +223,9 @@ class OracleSystem:
         lib().orc_momentum(self._s, ctypes.byref(p), out)
         return np.array(list(out))
 
-    def step_wcsph(self, nsteps, **kw):
+    def step_wcsph(self, nsteps, use_rho0=False, **kw):
         p = params(**kw)
-        return lib().orc_step_wcsph(self._s, ctypes.byref(p), int(nsteps))
+        return lib().orc_step_wcsph(self._s, ctypes.byref(p), int(nsteps), 1 if use_rho0 else 0)
 
 
 def brute_pairs(x, h):

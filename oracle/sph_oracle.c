@@ -589,6 +589,27 @@ static double energy_wcsph(const orc_sys* s, const orc_params* pr, int64_t i)
     return kinetic + potential + internal;
 }
 
+/* Energy whose internal part matches P = c^2 (rho - rho0_p) with the
+ * per-particle reference density of find_rho0! (tests/test_collision_2d.jl:
+ * 67-73): m c^2 (ln(rho/rho0_p) + rho0_p/rho - 1); kinetic and potential
+ * as examples/collapse_dry.jl:166-171. */
+static double energy_wcsph_rho0(const orc_sys* s, const orc_params* pr, int64_t i)
+{
+    const double* v = s->v + 3 * i;
+    const double* x = s->x + 3 * i;
+    double m = pr->m;
+    double vv = v[0] * v[0] + v[1] * v[1];
+    vv = vv + v[2] * v[2];
+    double gx = pr->g[0] * x[0] + pr->g[1] * x[1];
+    gx = gx + pr->g[2] * x[2];
+    double kinetic = 0.5 * m * vv;
+    double potential = -m * gx;
+    double r0 = s->rho0[i];
+    double internal = m * (pr->c * pr->c) *
+                      (log(fabs(s->rho[i] / r0)) + r0 / s->rho[i] - 1.0);
+    return kinetic + potential + internal;
+}
+
 /* tests/test_collision_2d.jl:96-100 */
 static double energy_collision(const orc_sys* s, const orc_params* pr, int64_t i)
 {
@@ -606,7 +627,8 @@ static double pairwise(const orc_sys* s, const orc_params* pr, int kind,
     if (hi - lo <= 1024) {
         double acc = 0.0;
         for (int64_t i = lo; i < hi; ++i)
-            acc += kind == 0 ? energy_wcsph(s, pr, i) : energy_collision(s, pr, i);
+            acc += kind == 0 ? energy_wcsph(s, pr, i)
+                             : (kind == 1 ? energy_collision(s, pr, i) : energy_wcsph_rho0(s, pr, i));
         return acc;
     }
     int64_t mid = lo + (hi - lo) / 2;
@@ -614,7 +636,8 @@ static double pairwise(const orc_sys* s, const orc_params* pr, int kind,
 }
 
 /* kind 0: WCSPH energy (collapse_dry.jl:166-171); kind 1: collision energy
- * (test_collision_2d.jl:96-100). Pairwise summation as Base.sum. */
+ * (test_collision_2d.jl:96-100); kind 2: WCSPH energy with the per-particle
+ * rho0.  Pairwise summation as Base.sum. */
 double orc_energy(const orc_sys* s, const orc_params* pr, int kind)
 {
     return pairwise(s, pr, kind, 0, s->n);
@@ -636,9 +659,11 @@ void orc_momentum(const orc_sys* s, const orc_params* pr, double* out3)
  *   reset_a!; internal_force!; accelerate! (coeff 1); move! (coeff 1).
  * Built from the operators above in the order of
  * tests/test_collision_2d.jl:106-116 with a full kick and drift.
+ * use_rho0 selects P = c^2 (rho - rho0_p) with the per-particle reference
+ * density (tests/test_collision_2d.jl:71-73) instead of the constant rho0.
  * Returns the accepted ordered pairs of the density passes summed over the
  * steps. */
-int64_t orc_step_wcsph(orc_sys* s, const orc_params* pr, int nsteps)
+int64_t orc_step_wcsph(orc_sys* s, const orc_params* pr, int nsteps, int use_rho0)
 {
     orc_params p1 = *pr;
     p1.coeff = 1.0;
@@ -647,7 +672,7 @@ int64_t orc_step_wcsph(orc_sys* s, const orc_params* pr, int nsteps)
         orc_create_cell_list(s);
         orc_reset_rho(s);
         pairs += orc_density_sum(s, &p1, 1);
-        orc_eos(s, &p1, 0);
+        orc_eos(s, &p1, use_rho0);
         orc_reset_a(s);
         orc_internal_force(s, &p1);
         orc_accelerate(s, &p1);

@@ -512,7 +512,7 @@ int sph_reduce(sph_system* s, int what, const sph_params* params, double* out)
 {
     if (!s || !params || !out) return fail(SPH_EINVAL, "null argument");
     if (what != SPH_REDUCE_ENERGY && what != SPH_REDUCE_ENERGY_COLLISION &&
-        what != SPH_REDUCE_MOMENTUM)
+        what != SPH_REDUCE_MOMENTUM && what != SPH_REDUCE_ENERGY_RHO0)
         return fail(SPH_EUNSUP, "unknown reduction");
     const Phys ph = make_phys(s, params);
     double res[3] = {0.0, 0.0, 0.0};
@@ -566,7 +566,9 @@ int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
 int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, int64_t* pairs)
 {
     if (!s || !params) return fail(SPH_EINVAL, "null argument");
-    if (scheme != SPH_SCHEME_WCSPH_SE) return fail(SPH_EUNSUP, "unknown scheme");
+    if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0)
+        return fail(SPH_EUNSUP, "unknown scheme");
+    const int eos = scheme == SPH_SCHEME_WCSPH_SE ? SPH_OP_FIND_PRESSURE : SPH_OP_FIND_PRESSURE_RHO0;
     if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
     int rc;
     if (s->cap == 0 && (rc = ensure_capacity(s, 1))) return rc;
@@ -584,7 +586,7 @@ int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, in
         if ((rc = run_pair(s, P_DENSITY, ph, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY)))
             return rc;
         prof_begin(s, SPH_KERNEL_UNARY, ev);
-        enqueue_unary(s, SPH_OP_FIND_PRESSURE, ph, s->prho2);
+        enqueue_unary(s, eos, ph, s->prho2);
         prof_end(s, SPH_KERNEL_UNARY, ev);
         if ((rc = run_pair(s, P_FORCE, ph, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE)))
             return rc;

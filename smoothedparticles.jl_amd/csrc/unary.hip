@@ -36,7 +36,9 @@ __global__ __launch_bounds__(256) void k_unary(int op, const int* meta, double* 
     }
     case SPH_OP_FIND_PRESSURE_RHO0: {  // tests/test_collision_2d.jl:71-73
         const double rho = UCOL(cols, cap, C_RHO, i);
-        UCOL(cols, cap, C_P, i) = (ph.c * ph.c) * (rho - UCOL(cols, cap, C_RHO0, i));
+        const double P = (ph.c * ph.c) * (rho - UCOL(cols, cap, C_RHO0, i));
+        UCOL(cols, cap, C_P, i) = P;
+        if (prho2) prho2[i] = P / (rho * rho);
         break;
     }
     case SPH_OP_FIND_PRESSURE_CONTINUITY: {  // examples/collapse_dry.jl:123-127
@@ -134,6 +136,19 @@ __device__ inline double energy_of(int what, const double* cols, long long cap, 
         const double potential = -ph.m * gx;
         const double internal =
             ph.m * (ph.c * ph.c) * (log(fabs(rho / ph.rho0)) + ph.rho0 / rho - 1.0);
+        return kinetic + potential + internal;
+    }
+    if (what == SPH_REDUCE_ENERGY_RHO0) {
+        // kinetic and potential of examples/collapse_dry.jl:166-171; internal
+        // energy of P = c^2 (rho - rho0_p)
+        const double x0 = UCOL(cols, cap, C_X, i), x1 = UCOL(cols, cap, C_Y, i),
+                     x2 = UCOL(cols, cap, C_Z, i);
+        double gx = ph.g[0] * x0 + ph.g[1] * x1;
+        gx = gx + ph.g[2] * x2;
+        const double r0 = UCOL(cols, cap, C_RHO0, i);
+        const double kinetic = 0.5 * ph.m * vv;
+        const double potential = -ph.m * gx;
+        const double internal = ph.m * (ph.c * ph.c) * (log(fabs(rho / r0)) + r0 / rho - 1.0);
         return kinetic + potential + internal;
     }
     // tests/test_collision_2d.jl:96-100

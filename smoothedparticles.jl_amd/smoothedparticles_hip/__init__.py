@@ -312,8 +312,10 @@ def _reduce(sys, what, nout, **params):
 
 def energy(sys, kind="wcsph", **params):
     """Total energy: kind "wcsph" = examples/collapse_dry.jl:166-171,
-    kind "collision" = tests/test_collision_2d.jl:96-100."""
-    what = {"wcsph": N.SPH_REDUCE_ENERGY, "collision": N.SPH_REDUCE_ENERGY_COLLISION}[kind]
+    kind "collision" = tests/test_collision_2d.jl:96-100, kind "wcsph_rho0" =
+    the WCSPH energy with the per-particle reference density rho0_p."""
+    what = {"wcsph": N.SPH_REDUCE_ENERGY, "collision": N.SPH_REDUCE_ENERGY_COLLISION,
+            "wcsph_rho0": N.SPH_REDUCE_ENERGY_RHO0}[kind]
     return _reduce(sys, what, 1, **params)[0]
 
 
@@ -336,10 +338,19 @@ def neighbour_pairs(sys):
 
 def step(sys, nsteps, scheme="wcsph_se", **params):
     """nsteps fused device-resident steps (density sum, EOS, internal force,
-    symplectic Euler kick and drift).  Returns the accepted ordered pairs
-    summed over the density passes."""
+    symplectic Euler kick and drift).  scheme "wcsph_se" uses the constant
+    rho0 in the EOS, "wcsph_se_rho0" the per-particle rho0 field.  Returns
+    the accepted ordered pairs summed over the density passes."""
+    sid = {"wcsph_se": N.SPH_SCHEME_WCSPH_SE, "wcsph_se_rho0": N.SPH_SCHEME_WCSPH_SE_RHO0}[scheme]
     p = make_params(**params)
     pairs = ctypes.c_int64()
-    check(lib().sph_step(sys._h, N.SPH_SCHEME_WCSPH_SE, ctypes.byref(p), int(nsteps),
-                         ctypes.byref(pairs)))
+    check(lib().sph_step(sys._h, sid, ctypes.byref(p), int(nsteps), ctypes.byref(pairs)))
     return pairs.value
+
+
+def init_reference_density(sys, **params):
+    """create_cell_list!(sys); apply!(sys, find_rho0!, self = true): the
+    per-particle reference density of tests/test_collision_2d.jl:123-124."""
+    create_cell_list(sys)
+    sys.set("rho0", 0.0)
+    apply(sys, find_rho0, self=True, **params)

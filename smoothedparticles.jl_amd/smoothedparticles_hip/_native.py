@@ -41,8 +41,10 @@ SPH_OP_MOVE = 22
 SPH_REDUCE_ENERGY = 0
 SPH_REDUCE_ENERGY_COLLISION = 1
 SPH_REDUCE_MOMENTUM = 2
+SPH_REDUCE_ENERGY_RHO0 = 3
 
 SPH_SCHEME_WCSPH_SE = 0
+SPH_SCHEME_WCSPH_SE_RHO0 = 1
 
 SPH_KERNEL_BUILD = 0
 SPH_KERNEL_DENSITY = 1
