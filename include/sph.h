@@ -177,6 +177,13 @@ int sph_key_geometry(sph_system* s, int64_t* key_phase, int64_t* key_lim,
 int sph_set_tile(sph_system* s, int targets_per_tile);
 int sph_autotune(sph_system* s, const sph_params* params, int* best);
 
+/* Options (no counterpart in the reference).  SPH_OPTION_PAIR_MASKS (default
+ * 1): a density pass records, per target and 64-candidate chunk, the set of
+ * accepted candidates, and later pair passes on the same cell list read it
+ * instead of re-testing candidates.  0 makes every pass test. */
+#define SPH_OPTION_PAIR_MASKS 0
+int sph_set_option(sph_system* s, int option, int value);
+
 /* Diagnostics (no counterpart in the reference): candidate particles the
  * pair kernels test per pass under the current cell list, i.e. for every
  * particle the particles of the cells key + key_diff it visits

@@ -212,7 +212,35 @@ PairCtx make_ctx(sph_system* s, const Phys& ph)
     c.cols = s->cols[s->cur];
     c.refidx = s->icols[s->cur] + I_REF * s->cap;
     c.prho2 = s->prho2;
+    c.masks = s->masks;
+    c.mask_mode = MASK_TEST;
     return c;
+}
+
+// Grow the pair-mask buffer to what the current tile table needs and mark
+// it usable; called at host sync points (the fused step tests instead of
+// loading while the buffer is too small).
+int ensure_masks(sph_system* s)
+{
+    if (!s->use_masks) return 0;
+    unsigned long long need = 0;
+    HIPC(hipMemcpyAsync(&need, s->mask_total, sizeof(need), hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    if ((long long)need > s->mask_cap) {
+        dfree(s->masks);
+        s->mask_cap = 0;
+        const long long cap = (long long)need + (long long)need / 4 + 4096;
+        if (dalloc(&s->masks, cap)) {
+            s->masks = nullptr;  // no masks: the kernels keep testing
+            g_err.clear();
+            return 0;
+        }
+        s->mask_cap = cap;
+    }
+    const int ok = 1;
+    HIPC(hipMemcpyAsync(s->meta + META_MASK_OK, &ok, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    return 0;
 }
 
 // Grid of the pair kernels: the tiles are taken from a work queue, so a
@@ -224,12 +252,13 @@ int pair_grid(const sph_system* s, long long ntiles)
 }
 
 int run_pair(sph_system* s, int op, const Phys& ph, int self, int overwrite,
-             unsigned long long* counter, long long ntiles, int prof_kernel)
+             unsigned long long* counter, long long ntiles, int prof_kernel, int mask_mode)
 {
     PairCtx c = make_ctx(s, ph);
     c.self = self;
     c.overwrite = overwrite;
     c.pair_counter = counter;
+    c.mask_mode = s->use_masks ? mask_mode : MASK_TEST;
     hipEvent_t ev[2];
     prof_begin(s, prof_kernel, ev);
     reset_queue(s);
@@ -320,6 +349,11 @@ int sph_create(sph_system** out, double h, const double* bmin, const double* bma
     s->slot = s->tmp_idx = s->tmp_key = s->tmp_ref = s->perm = s->inv = nullptr;
     s->tiles = nullptr;
     s->prho2 = s->stage = s->red_part = nullptr;
+    s->cell_chunks = nullptr;
+    s->mask_start = s->scan_part64 = s->masks = s->mask_total = nullptr;
+    s->mask_cap = 0;
+    s->masks_valid = false;
+    s->use_masks = true;
     s->prof = false;
     s->step_pairs[0] = s->step_pairs[1] = 0;
     int rc;
@@ -332,7 +366,9 @@ int sph_create(sph_system** out, double h, const double* bmin, const double* bma
     if ((rc = dalloc(&s->meta, META_SLOTS)) || (rc = dalloc(&s->pair_counter, 4)) ||
         (rc = dalloc(&s->cnt, km + 1)) || (rc = dalloc(&s->cell_start, km + 2)) ||
         (rc = dalloc(&s->tile_start, km + 2)) || (rc = dalloc(&s->scan_part, s->scan_part_n)) ||
-        (rc = dalloc(&s->red_out, 4))) {
+        (rc = dalloc(&s->red_out, 4)) || (rc = dalloc(&s->cell_chunks, km + 1)) ||
+        (rc = dalloc(&s->mask_start, km + 2)) || (rc = dalloc(&s->scan_part64, s->scan_part_n)) ||
+        (rc = dalloc(&s->mask_total, 1))) {
         sph_destroy(s);
         return rc;
     }
@@ -360,6 +396,8 @@ int sph_destroy(sph_system* s)
     free_particle_arrays(s);
     dfree(s->meta); dfree(s->pair_counter); dfree(s->cnt); dfree(s->cell_start);
     dfree(s->tile_start); dfree(s->scan_part); dfree(s->red_out);
+    dfree(s->cell_chunks); dfree(s->mask_start); dfree(s->scan_part64); dfree(s->masks);
+    dfree(s->mask_total);
     if (s->meta_host) hipHostFree(s->meta_host);
     for (auto e : s->ev_pool) hipEventDestroy(e);
     for (auto& e : s->ev_open) {
@@ -390,7 +428,7 @@ int sph_append(sph_system* s, int64_t n, const double* x, const double* v, const
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
     s->n += n;
-    s->cells_valid = false;
+    s->cells_valid = s->masks_valid = false;
     return 0;
 }
 
@@ -413,7 +451,7 @@ int sph_set_field(sph_system* s, int field, const double* host, int64_t n)
     enqueue_field_in(s, col, nc, s->stage);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
-    if (field == SPH_FIELD_X) s->cells_valid = false;
+    if (field == SPH_FIELD_X) s->cells_valid = s->masks_valid = false;
     return 0;
 }
 
@@ -452,7 +490,9 @@ int sph_build_cells(sph_system* s, int64_t* n_out)
             if ((rc = sync_meta(s))) return rc;
         }
     }
+    if ((rc = ensure_masks(s))) return rc;
     s->cells_valid = true;
+    s->masks_valid = false;
     if (n_out) *n_out = s->n;
     return 0;
 }
@@ -480,9 +520,13 @@ int sph_apply_binary(sph_system* s, int op, const sph_params* params, int self)
     // density sums; for the other operators x_pp = v_pp = 0 and rho_p - rho_p
     // = 0, so the term is zero.
     const int dens = pop == P_DENSITY || pop == P_DENSITY_RHO0;
-    int rc = run_pair(s, pop, ph, dens && self, 0, nullptr, s->meta_host[META_NTILES], kid);
+    // A density pass records the pair masks of this cell list; the passes
+    // after it read them.
+    const int mode = dens ? MASK_TEST_STORE : (s->masks_valid ? MASK_LOAD : MASK_TEST);
+    int rc = run_pair(s, pop, ph, dens && self, 0, nullptr, s->meta_host[META_NTILES], kid, mode);
     if (rc) return rc;
     HIPC(hipStreamSynchronize(s->stream));
+    if (dens) s->masks_valid = s->use_masks && s->masks != nullptr;
     return 0;
 }
 
@@ -497,7 +541,7 @@ int sph_apply_unary(sph_system* s, int op, const sph_params* params)
     prof_end(s, SPH_KERNEL_UNARY, ev);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
-    if (op == SPH_OP_MOVE) s->cells_valid = false;
+    if (op == SPH_OP_MOVE) s->cells_valid = s->masks_valid = false;
     return 0;
 }
 
@@ -537,7 +581,8 @@ int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
     const Phys ph = make_phys(s, &zero);
     unsigned long long total = 0;
     HIPC(hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream));
-    int rc = run_pair(s, P_COUNT, ph, 0, 0, s->pair_counter, s->meta_host[META_NTILES], SPH_KERNEL_DENSITY);
+    int rc = run_pair(s, P_COUNT, ph, 0, 0, s->pair_counter, s->meta_host[META_NTILES],
+                      SPH_KERNEL_DENSITY, MASK_TEST);
     if (rc) return rc;
     HIPC(hipMemcpyAsync(&total, s->pair_counter, sizeof(total), hipMemcpyDeviceToHost, s->stream));
     HIPC(hipStreamSynchronize(s->stream));
@@ -548,7 +593,7 @@ int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
     if ((rc = dalloc(&dpairs, 2 * ncopy))) return rc;
     HIPC(hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream));
     PairCtx c = make_ctx(s, ph);
-    c.pair_counter = s->pair_counter;
+    c.emit_counter = s->pair_counter;
     c.pair_out = dpairs;
     c.pair_cap = ncopy;
     reset_queue(s);
@@ -583,12 +628,14 @@ int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, in
         prof_begin(s, SPH_KERNEL_BUILD, ev);
         enqueue_build(s);
         prof_end(s, SPH_KERNEL_BUILD, ev);
-        if ((rc = run_pair(s, P_DENSITY, ph, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY)))
+        if ((rc = run_pair(s, P_DENSITY, ph, 1, 1, s->pair_counter, tiles_bound,
+                           SPH_KERNEL_DENSITY, MASK_TEST_STORE)))
             return rc;
         prof_begin(s, SPH_KERNEL_UNARY, ev);
         enqueue_unary(s, eos, ph, s->prho2);
         prof_end(s, SPH_KERNEL_UNARY, ev);
-        if ((rc = run_pair(s, P_FORCE, ph, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE)))
+        if ((rc = run_pair(s, P_FORCE, ph, 0, 1, s->pair_counter + 1, tiles_bound,
+                           SPH_KERNEL_FORCE, MASK_LOAD)))
             return rc;
         prof_begin(s, SPH_KERNEL_UNARY, ev);
         enqueue_kick_drift(s, ph);
@@ -598,7 +645,8 @@ int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, in
     HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 2 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s->stream));
     if ((rc = sync_meta(s))) return rc;
-    s->cells_valid = false;  // positions moved after the last build
+    s->cells_valid = s->masks_valid = false;  // positions moved after the last build
+    if ((rc = ensure_masks(s))) return rc;    // size the masks for the next call
     if (pairs) *pairs = (int64_t)s->step_pairs[0];
     return 0;
 }
@@ -650,10 +698,12 @@ int sph_set_tile(sph_system* s, int T)
     s->tile_auto = 0;
     if (T != s->tile_T) {
         s->tile_T = T;
+        s->masks_valid = false;
         if (s->cells_valid) {
             enqueue_tiles(s);
             HIPC(hipGetLastError());
             int rc = sync_meta(s);
+            if (!rc) rc = ensure_masks(s);
             if (rc) return rc;
         }
     }
@@ -683,9 +733,11 @@ int sph_autotune(sph_system* s, const sph_params* params, int* best)
         float tot = 0.0f;
         for (int rep = 0; rep < 3; ++rep) {
             HIPC(hipEventRecord(e0, s->stream));
-            if ((rc = run_pair(s, P_DENSITY, ph, 0, 0, nullptr, s->meta_host[META_NTILES], SPH_KERNEL_DENSITY)))
+            if ((rc = run_pair(s, P_DENSITY, ph, 0, 0, nullptr, s->meta_host[META_NTILES],
+                               SPH_KERNEL_DENSITY, MASK_TEST_STORE)))
                 return rc;
-            if ((rc = run_pair(s, P_FORCE, ph, 0, 0, nullptr, s->meta_host[META_NTILES], SPH_KERNEL_FORCE)))
+            if ((rc = run_pair(s, P_FORCE, ph, 0, 0, nullptr, s->meta_host[META_NTILES],
+                               SPH_KERNEL_FORCE, MASK_LOAD)))
                 return rc;
             HIPC(hipEventRecord(e1, s->stream));
             HIPC(hipEventSynchronize(e1));
@@ -702,8 +754,22 @@ int sph_autotune(sph_system* s, const sph_params* params, int* best)
     hipEventDestroy(e1);
     int rc = sph_set_tile(s, best_T);
     if (rc) return rc;
+    s->masks_valid = false;  // the timing passes ran with a zero mass
     if (best) *best = best_T;
     return 0;
+}
+
+int sph_set_option(sph_system* s, int option, int value)
+{
+    if (!s) return fail(SPH_EINVAL, "null system");
+    switch (option) {
+    case SPH_OPTION_PAIR_MASKS:
+        s->use_masks = value != 0;
+        s->masks_valid = false;
+        if (s->use_masks && s->cells_valid) return ensure_masks(s);
+        return 0;
+    }
+    return fail(SPH_EINVAL, "unknown option");
 }
 
 int sph_candidates(sph_system* s, int64_t* all, int64_t* fluid)

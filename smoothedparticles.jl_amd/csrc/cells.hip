@@ -84,26 +84,31 @@ __global__ __launch_bounds__(256) void k_keys(Geo g, const double* __restrict__ 
 }
 
 // ------------------------------------------------------------------ scan --
-__device__ inline unsigned wave_incl_scan(unsigned v)
+// Exclusive device-wide scan, three launches: per-block sums, one block
+// scanning the block sums, per-block scans plus offset.  V is unsigned or
+// unsigned long long.
+template <class V>
+__device__ inline V wave_incl_scan(V v)
 {
     const int lane = __lane_id();
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const unsigned t = __shfl_up(v, d, 64);
+        const V t = __shfl_up(v, d, 64);
         if (lane >= d) v += t;
     }
     return v;
 }
 
 // Inclusive scan over a 256-thread block; `total` gets the block sum.
-__device__ inline unsigned block_incl_scan(unsigned v, unsigned& total)
+template <class V>
+__device__ inline V block_incl_scan(V v, V& total)
 {
-    __shared__ unsigned wsum[4];
-    const unsigned w = wave_incl_scan(v);
+    __shared__ V wsum[4];
+    const V w = wave_incl_scan(v);
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 63) wsum[wid] = w;
     __syncthreads();
-    unsigned off = 0;
+    V off = 0;
     for (int i = 0; i < wid; ++i) off += wsum[i];
     total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
@@ -111,73 +116,93 @@ __device__ inline unsigned block_incl_scan(unsigned v, unsigned& total)
 }
 
 struct SrcCount {
+    using V = unsigned;
     const unsigned* c;
-    __device__ unsigned operator()(int i) const { return c[i]; }
+    __device__ V operator()(int i) const { return c[i]; }
 };
+// Tiles of a cell: ceil(count / T).
 struct SrcTiles {
+    using V = unsigned;
     const int* cs;
     int km;
     int T;
-    __device__ unsigned operator()(int i) const
+    __device__ V operator()(int i) const
     {
         if (i >= km) return 0u;
         const int n = cs[i + 1] - cs[i];
         return (unsigned)((n + T - 1) / T);
     }
 };
+// Mask words of a cell: tiles * chunks per target * T.
+struct SrcMaskWords {
+    using V = unsigned long long;
+    const int* tile_start;
+    const int* chunks;
+    int km;
+    int T;
+    __device__ V operator()(int i) const
+    {
+        if (i >= km) return 0ull;
+        const long long nt = tile_start[i + 1] - tile_start[i];
+        return (V)(nt * chunks[i] * T);
+    }
+};
 
 template <class Src>
-__global__ __launch_bounds__(256) void k_scan_reduce(Src src, int n, unsigned* part)
+__global__ __launch_bounds__(256) void k_scan_reduce(Src src, int n, typename Src::V* part)
 {
+    using V = typename Src::V;
     const int base = blockIdx.x * SCAN_TILE;
-    unsigned s = 0;
+    V s = 0;
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int i = base + it * 256 + threadIdx.x;
         if (i < n) s += src(i);
     }
-    unsigned total;
+    V total;
     block_incl_scan(s, total);
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(256) void k_scan_partials(unsigned* part, int nb)
+template <class V>
+__global__ __launch_bounds__(256) void k_scan_partials(V* part, int nb)
 {
-    unsigned carry = 0;
+    V carry = 0;
     for (int b = 0; b < nb; b += 256) {
         const int i = b + threadIdx.x;
-        const unsigned v = i < nb ? part[i] : 0u;
-        unsigned total;
-        const unsigned inc = block_incl_scan(v, total);
+        const V v = i < nb ? part[i] : (V)0;
+        V total;
+        const V inc = block_incl_scan(v, total);
         if (i < nb) part[i] = carry + inc - v;
         carry += total;
     }
 }
 
 // out[i] = exclusive prefix for i < n, out[n] = total.
-template <class Src>
-__global__ __launch_bounds__(256) void k_scan_down(Src src, int n, const unsigned* part,
-                                                   int* out)
+template <class Src, class O>
+__global__ __launch_bounds__(256) void k_scan_down(Src src, int n, const typename Src::V* part,
+                                                   O* out)
 {
+    using V = typename Src::V;
     const int base = blockIdx.x * SCAN_TILE;
-    unsigned carry = part[blockIdx.x];
+    V carry = part[blockIdx.x];
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int i = base + it * 256 + threadIdx.x;
-        const unsigned v = i < n ? src(i) : 0u;
-        unsigned total;
-        const unsigned inc = block_incl_scan(v, total);
-        if (i < n) out[i] = (int)(carry + inc - v);
-        if (i == n - 1) out[n] = (int)(carry + inc);
+        const V v = i < n ? src(i) : (V)0;
+        V total;
+        const V inc = block_incl_scan(v, total);
+        if (i < n) out[i] = (O)(carry + inc - v);
+        if (i == n - 1) out[n] = (O)(carry + inc);
         carry += total;
     }
 }
 
-template <class Src>
-static void scan(sph_system* s, Src src, int n, int* out)
+template <class Src, class O>
+static void scan(sph_system* s, Src src, int n, O* out, typename Src::V* part)
 {
     const int nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan_reduce<<<nb, 256, 0, s->stream>>>(src, n, s->scan_part);
-    k_scan_partials<<<1, 256, 0, s->stream>>>(s->scan_part, nb);
-    k_scan_down<<<nb, 256, 0, s->stream>>>(src, n, s->scan_part, out);
+    k_scan_reduce<<<nb, 256, 0, s->stream>>>(src, n, part);
+    k_scan_partials<<<1, 256, 0, s->stream>>>(part, nb);
+    k_scan_down<<<nb, 256, 0, s->stream>>>(src, n, part, out);
 }
 
 long long scan_partials_needed(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
@@ -266,17 +291,50 @@ __global__ void k_remove_seq(const int* meta, int* ref, int* inv)
     }
 }
 
-__global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_start,
-                                               const int* __restrict__ tile_start, int key_max,
-                                               int T, int2* tiles, int* meta)
+// 64-candidate chunks one target of cell c walks (same loop as the pair
+// kernels: per stencil row, ceil(row length / 64)).
+template <int ROWS>
+__global__ __launch_bounds__(256) void k_cell_chunks(Geo g, const int* __restrict__ cell_start,
+                                                     int* chunks)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c == 0) meta[META_NTILES] = tile_start[key_max];
+    if (c >= g.key_max) return;
+    int n = 0;
+    if (cell_start[c + 1] > cell_start[c]) {
+        for (int row = 0; row < ROWS; ++row) {
+            int j0, j1;
+            if (row_range<ROWS>(g, cell_start, c, row, j0, j1)) n += (j1 - j0 + 63) / 64;
+        }
+    }
+    chunks[c] = n;
+}
+
+// Tile table: tile k of cell c covers targets cell_start[c] + k*T ...; its
+// pair masks start at mask_start[c] + k * chunks[c] * T.
+__global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_start,
+                                               const int* __restrict__ tile_start,
+                                               const int* __restrict__ chunks,
+                                               const unsigned long long* __restrict__ mask_start,
+                                               int key_max, int T, long long mask_cap, int4* tiles,
+                                               int* meta, unsigned long long* mask_total)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c == 0) {
+        meta[META_NTILES] = tile_start[key_max];
+        const unsigned long long need = mask_start[key_max];
+        *mask_total = need;
+        meta[META_MASK_OK] = (long long)need <= mask_cap ? 1 : 0;
+    }
     if (c >= key_max) return;
     const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
     if (nt > 0) atomicAdd(&meta[META_NCELLS], 1);
     const int cs = cell_start[c];
-    for (int t = 0; t < nt; ++t) tiles[t0 + t] = make_int2(c, cs + t * T);
+    const long long words = (long long)chunks[c] * T;
+    for (int t = 0; t < nt; ++t) {
+        const unsigned long long base = mask_start[c] + (unsigned long long)(t * words);
+        tiles[t0 + t] = make_int4(c, cs + t * T, (int)(unsigned)(base & 0xffffffffull),
+                                  (int)(base >> 32));
+    }
 }
 
 // ------------------------------------------------------------ host side --
@@ -285,9 +343,17 @@ static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 
 void enqueue_tiles(sph_system* s)
 {
     const Geo& g = s->geo;
-    scan(s, SrcTiles{s->cell_start, g.key_max, s->tile_T}, g.key_max + 1, s->tile_start);
-    k_tiles<<<nblocks(g.key_max + 1, 256), 256, 0, s->stream>>>(
-        s->cell_start, s->tile_start, g.key_max, s->tile_T, s->tiles, s->meta);
+    const unsigned nbc = nblocks(g.key_max + 1, 256);
+    scan(s, SrcTiles{s->cell_start, g.key_max, s->tile_T}, g.key_max + 1, s->tile_start,
+         s->scan_part);
+    if (g.dim == 2) k_cell_chunks<3><<<nbc, 256, 0, s->stream>>>(g, s->cell_start, s->cell_chunks);
+    else k_cell_chunks<9><<<nbc, 256, 0, s->stream>>>(g, s->cell_start, s->cell_chunks);
+    scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, g.key_max, s->tile_T}, g.key_max + 1,
+         s->mask_start, s->scan_part64);
+    k_tiles<<<nbc, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
+                                        s->mask_start, g.key_max, s->tile_T,
+                                        s->masks ? s->mask_cap : 0, s->tiles, s->meta,
+                                        s->mask_total);
 }
 
 void enqueue_build(sph_system* s)
@@ -301,7 +367,7 @@ void enqueue_build(sph_system* s)
     k_fill_u32<<<nblocks(g.key_max + 1, 256), 256, 0, st>>>(s->cnt, 0u, g.key_max + 1);
     k_keys<<<nb, 256, 0, st>>>(g, c + C_X * cap, c + C_Y * cap, c + C_Z * cap, s->meta,
                                ic + I_KEY * cap, s->slot, s->cnt);
-    scan(s, SrcCount{s->cnt}, g.key_max + 1, s->cell_start);
+    scan(s, SrcCount{s->cnt}, g.key_max + 1, s->cell_start, s->scan_part);
     k_scatter<<<nb, 256, 0, st>>>(s->meta, ic + I_KEY * cap, s->slot, ic + I_REF * cap,
                                   s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref);
     k_rank<<<nb, 256, 0, st>>>(s->meta, s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref,

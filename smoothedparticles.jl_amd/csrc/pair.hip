@@ -11,8 +11,13 @@
 // small grids visit aliased cells as often as the reference does.  Since
 // the particles are cell-sorted, each row is one contiguous particle range.
 // The wave streams it through LDS in chunks of 64 (one coalesced load per
-// lane), every lane then tests the chunk entries of its slice, and at the
-// end the S slices of a target are combined with wave shuffles.
+// lane).  Each chunk is handled in two phases:
+//   1. mask: every lane tests the chunk entries jj = s (mod S) and sets bit
+//      jj of a 64-bit mask for each accepted candidate -- or, after a
+//      density pass on the same cell list, loads that pass's mask;
+//   2. interact: every lane walks the set bits of its mask, so the pair
+//      arithmetic runs once per accepted pair, not once per candidate.
+// At the end the S slices of a target are combined with wave shuffles.
 //
 // Predicate (src/core.jl:104-105, src/algebra.jl:49-60): accept q != p when
 // RN(sqrt((d1*d1 + d2*d2) + d3*d3)) <= h.  sqrt is correctly rounded and
@@ -21,7 +26,7 @@
 // (-ffp-contract=off), i.e. bit-exactly as the reference.  The mixed
 // kernels evaluate s in FP32 on tile-relative coordinates and fall back to
 // the exact FP64 test inside a guard band around s_thr, so their neighbour
-// sets are the reference's too.
+// sets (and masks) are the reference's too.
 #include "sph_internal.h"
 
 namespace sph {
@@ -50,49 +55,58 @@ __device__ inline void two_sum_add(float& s, float& e, float x)
     s = t;
 }
 
-// ================================================================ FP64 ====
-
-struct LdsPos64 {
-    double x[64], y[64], z[64];
-};
-struct LdsForce64 {
-    double x[64], y[64], z[64], vx[64], vy[64], vz[64], pr[64], rho[64];
-};
-struct LdsCont64 {
-    double x[64], y[64], z[64], vx[64], vy[64], vz[64], rho[64];
-};
-struct LdsPres64 {
-    double x[64], y[64], z[64], P[64];
-};
-
-template <class L>
-__device__ inline void stage_pos64(const PairCtx& c, int q, int l, L& s)
+// Bits jj with jj % S == 0 (S = 1, 2, 4, 8).
+__device__ inline unsigned long long slice_pattern(int S)
 {
-    s.x[l] = COL(c, C_X, q);
-    s.y[l] = COL(c, C_Y, q);
-    s.z[l] = COL(c, C_Z, q);
+    return ~0ull / ((1ull << S) - 1ull);
 }
 
-// rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0); also the
-// accepted-pair counter of the metric.
+// Tiles come from a work queue (one atomic per tile); every wave leaves
+// the loop once the queue passes the tile count.
+__device__ inline int next_tile(const PairCtx& c)
+{
+    int t = 0;
+    if (__lane_id() == 0) t = atomicAdd(c.queue, 1);
+    return __shfl(t, 0);
+}
+
+__device__ inline long long mask_base(const int4& td)
+{
+    return (long long)(unsigned)td.z | ((long long)td.w << 32);
+}
+
+// ================================================================ FP64 ====
+
+// LDS staging of one 64-candidate chunk (FP64): xy, (z, rho), (vx, vy),
+// (vz, aux) where aux is P/rho^2 (force) or P (force with rho0).
+struct Lds64 {
+    double2 xy[64], zr[64], v01[64], v2a[64];
+};
+
+__device__ inline void stage_pos64(const PairCtx& c, int q, int l, Lds64& s)
+{
+    s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
+    s.zr[l].x = COL(c, C_Z, q);
+}
+
+// rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0).
 template <int OUT>
 struct Density64 {
-    using Lds = LdsPos64;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         double x, y, z;
     };
     struct Acc {
         double w;
-        unsigned n;
     };
     static __device__ void load(const PairCtx& c, int p, Tgt& t)
     {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s) { stage_pos64(c, q, l, s); }
-    static __device__ void init(Acc& a) { a.w = 0.0; a.n = 0; }
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int,
+    static __device__ bool active(const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s) { stage_pos64(c, q, l, s); }
+    static __device__ void init(Acc& a) { a.w = 0.0; }
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds64&, int, int, int,
                                     double, double, double, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -100,20 +114,16 @@ struct Density64 {
         const double u = 1.0 - x;
         const double u2 = u * u;
         a.w += u2 * u2 * (1.0 + 4.0 * x);
-        a.n += 1;
     }
-    static __device__ void reduce(Acc& a, int T)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) a.w += __shfl_xor(a.w, o);
-    }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
-    {
+        if (!write) return;
         double v = c.overwrite ? 0.0 : COL(c, OUT, p);
         v = v + c.ph.kW * a.w;
         if (c.self) v = v + c.ph.kW;
         COL(c, OUT, p) = v;
     }
-    static __device__ unsigned pairs(const Acc& a) { return a.n; }
 };
 
 // internal_force! (examples/collapse_dry.jl:135-141) + Monaghan viscosity.
@@ -122,14 +132,13 @@ struct Density64 {
 //   viscosity a += -kdW u^3 Pi x_pq,  Pi = avk (v.x)/(rhobar (r^2 + eps)), v.x < 0
 //   laminar   a += 2 kdW mu / rho0^2 u^3 v_pq
 struct Force64 {
-    using Lds = LdsForce64;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         double x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
     };
     struct Acc {
         double ax, ay, az, lx, ly, lz;
-        unsigned n;
     };
     static __device__ void load(const PairCtx& c, int p, Tgt& t)
     {
@@ -139,50 +148,44 @@ struct Force64 {
         t.rho = COL(c, C_RHO, p);
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt& t) { return t.fluid; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    static __device__ bool active(const Tgt& t) { return t.fluid; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
     {
-        stage_pos64(c, q, l, s);
-        s.vx[l] = COL(c, C_VX, q); s.vy[l] = COL(c, C_VY, q); s.vz[l] = COL(c, C_VZ, q);
-        s.pr[l] = c.prho2[q];
-        s.rho[l] = COL(c, C_RHO, q);
+        s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
+        s.zr[l] = make_double2(COL(c, C_Z, q), COL(c, C_RHO, q));
+        s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
+        s.v2a[l] = make_double2(COL(c, C_VZ, q), c.prho2[q]);
     }
-    static __device__ void init(Acc& a)
-    {
-        a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0;
-        a.n = 0;
-    }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+    static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
                                     int, double dx, double dy, double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
         const double x = r * c.g.inv_h;
         const double u = 1.0 - x;
         const double u3 = u * u * u;
-        double A = t.pr + s.pr[jj];
-        const double dvx = t.vx - s.vx[jj], dvy = t.vy - s.vy[jj], dvz = t.vz - s.vz[jj];
+        const double2 v01 = s.v01[jj], v2a = s.v2a[jj];
+        double A = t.pr + v2a.y;
+        const double dvx = t.vx - v01.x, dvy = t.vy - v01.y, dvz = t.vz - v2a.x;
         if (c.ph.avk != 0.0) {
             double vx = dvx * dx + dvy * dy;
             vx = vx + dvz * dz;
             if (vx < 0.0) {
-                const double rhobar = 0.5 * (t.rho + s.rho[jj]);
+                const double rhobar = 0.5 * (t.rho + s.zr[jj].y);
                 A = A + c.ph.avk * vx / (rhobar * (r * r + c.ph.eps));
             }
         }
         const double f = u3 * A;
         a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
         if (c.ph.mu != 0.0) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
-        a.n += 1;
     }
-    static __device__ void reduce(Acc& a, int T)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) {
             a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
             a.lx += __shfl_xor(a.lx, o); a.ly += __shfl_xor(a.ly, o); a.lz += __shfl_xor(a.lz, o);
         }
-    }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
-    {
+        if (!write) return;
         const double kp = -c.ph.kdW;
         const double kl = 2.0 * c.ph.kdW * c.ph.mu / (c.ph.rho0 * c.ph.rho0);
         double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
@@ -192,12 +195,11 @@ struct Force64 {
         if (c.ph.mu != 0.0) { ax = ax + kl * a.lx; ay = ay + kl * a.ly; az = az + kl * a.lz; }
         COL(c, C_AX, p) = ax; COL(c, C_AY, p) = ay; COL(c, C_AZ, p) = az;
     }
-    static __device__ unsigned pairs(const Acc& a) { return a.n; }
 };
 
 // internal_force! of tests/test_collision_2d.jl:75-78 (P/rho0^2, all targets).
 struct ForceRho064 {
-    using Lds = LdsPres64;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         double x, y, z, P;
     };
@@ -208,44 +210,41 @@ struct ForceRho064 {
     {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p); t.P = COL(c, C_P, p);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    static __device__ bool active(const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
     {
         stage_pos64(c, q, l, s);
-        s.P[l] = COL(c, C_P, q);
+        s.v2a[l].y = COL(c, C_P, q);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
                                     int, double dx, double dy, double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
         const double u = 1.0 - r * c.g.inv_h;
-        const double f = u * u * u * (t.P + s.P[jj]);
+        const double f = u * u * u * (t.P + s.v2a[jj].y);
         a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
     }
-    static __device__ void reduce(Acc& a, int T)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) {
             a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
         }
-    }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
-    {
+        if (!write) return;
         const double k = -c.ph.kdW / (c.ph.rho0 * c.ph.rho0);
-        double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
-        double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
-        double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
+        const double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
+        const double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
+        const double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
         COL(c, C_AX, p) = ax + k * a.ax;
         COL(c, C_AY, p) = ay + k * a.ay;
         COL(c, C_AZ, p) = az + k * a.az;
     }
-    static __device__ unsigned pairs(const Acc&) { return 0; }
 };
 
 // balance_of_mass! (examples/collapse_dry.jl:112-115):
 //   Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q))
 struct Continuity64 {
-    using Lds = LdsCont64;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         double x, y, z, vx, vy, vz, rho;
     };
@@ -258,101 +257,70 @@ struct Continuity64 {
         t.vx = COL(c, C_VX, p); t.vy = COL(c, C_VY, p); t.vz = COL(c, C_VZ, p);
         t.rho = COL(c, C_RHO, p);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s)
+    static __device__ bool active(const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
     {
-        stage_pos64(c, q, l, s);
-        s.vx[l] = COL(c, C_VX, q); s.vy[l] = COL(c, C_VY, q); s.vz[l] = COL(c, C_VZ, q);
-        s.rho[l] = COL(c, C_RHO, q);
+        s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
+        s.zr[l] = make_double2(COL(c, C_Z, q), COL(c, C_RHO, q));
+        s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
+        s.v2a[l].x = COL(c, C_VZ, q);
     }
     static __device__ void init(Acc& a) { a.d = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int,
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
                                     int, double dx, double dy, double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
         const double u = 1.0 - r * c.g.inv_h;
-        double d = dx * (t.vx - s.vx[jj]) + dy * (t.vy - s.vy[jj]);
-        d = d + dz * (t.vz - s.vz[jj]);
-        a.d += u * u * u * (d + 2.0 * c.ph.nu * (t.rho - s.rho[jj]));
+        const double2 v01 = s.v01[jj];
+        double d = dx * (t.vx - v01.x) + dy * (t.vy - v01.y);
+        d = d + dz * (t.vz - s.v2a[jj].x);
+        a.d += u * u * u * (d + 2.0 * c.ph.nu * (t.rho - s.zr[jj].y));
     }
-    static __device__ void reduce(Acc& a, int T)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) a.d += __shfl_xor(a.d, o);
-    }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a)
-    {
-        double v = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
+        if (!write) return;
+        const double v = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
         COL(c, C_DRHO, p) = v + c.ph.kdW * a.d;
     }
-    static __device__ unsigned pairs(const Acc&) { return 0; }
 };
 
 // Accepted-pair count (COUNT) or emission of (refidx p, refidx q) (EMIT).
 template <bool EMIT>
 struct Pairs64 {
-    using Lds = LdsPos64;
+    static constexpr bool INTERACT = EMIT;
     struct Tgt {
         double x, y, z;
     };
     struct Acc {
-        unsigned n;
+        int dummy;
     };
     static __device__ void load(const PairCtx& c, int p, Tgt& t)
     {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds& s) { stage_pos64(c, q, l, s); }
-    static __device__ void init(Acc& a) { a.n = 0; }
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int q,
-                                    double, double, double, double, Acc& a)
+    static __device__ bool active(const Tgt&) { return true; }
+    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s) { stage_pos64(c, q, l, s); }
+    static __device__ void init(Acc&) {}
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds64&, int, int p, int q,
+                                    double, double, double, double, Acc&)
     {
-        if (EMIT) {
-            const unsigned long long k = atomicAdd(c.pair_counter, 1ull);
-            if ((long long)k < c.pair_cap) {
-                c.pair_out[2 * k] = c.refidx[p];
-                c.pair_out[2 * k + 1] = c.refidx[q];
-            }
-        } else {
-            a.n += 1;
+        const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
+        if ((long long)k < c.pair_cap) {
+            c.pair_out[2 * k] = c.refidx[p];
+            c.pair_out[2 * k + 1] = c.refidx[q];
         }
     }
-    static __device__ void reduce(Acc&, int) {}
-    static __device__ void store(const PairCtx&, int, const Tgt&, const Acc&) {}
-    static __device__ unsigned pairs(const Acc& a) { return EMIT ? 0u : a.n; }
+    static __device__ void finish(const PairCtx&, int, const Tgt&, Acc&, int, bool) {}
 };
 
-// Row r of the 3 (2D) or 9 (3D) stencil rows: particle range [j0, j1).
-template <int ROWS>
-__device__ inline bool row_range(const PairCtx& c, int cell, int row, int& j0, int& j1)
-{
-    const int dj = row % 3 - 1;
-    const int dk = ROWS == 3 ? 0 : row / 3 - 1;
-    const long long mid = (long long)cell + (long long)c.g.L1 * dj +
-                          (long long)c.g.L1 * c.g.L2 * dk;
-    long long lo = mid - 1, hi = mid + 1;
-    if (lo < 0) lo = 0;
-    if (hi > c.g.key_max - 1) hi = c.g.key_max - 1;
-    if (lo > hi) return false;
-    j0 = c.cell_start[lo];
-    j1 = c.cell_start[hi + 1];
-    return j0 < j1;
-}
-
-// Tiles come from a work queue (one atomic per tile); every wave leaves
-// the loop once the queue passes the tile count.
-__device__ inline int next_tile(const PairCtx& c)
-{
-    int t = 0;
-    if (__lane_id() == 0) t = atomicAdd(c.queue, 1);
-    return __shfl(t, 0);
-}
-
 template <int ROWS, bool FLAT, class Op>
-__device__ inline void tile64(const PairCtx& c, int tile, typename Op::Lds& lds)
+__device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
+                              Lds64& lds)
 {
-    const int2 td = c.tiles[tile];
+    const int4 td = c.tiles[tile];
     const int cell = td.x, first = td.y;
+    const long long mbase = mask_base(td);
     const int cend = c.cell_start[cell + 1];
     const int lane = threadIdx.x;
     const int T = c.T, S = 64 / T;
@@ -361,46 +329,76 @@ __device__ inline void tile64(const PairCtx& c, int tile, typename Op::Lds& lds)
     const bool valid = p < cend;
     typename Op::Tgt tg;
     Op::load(c, valid ? p : first, tg);
-    const bool act = valid && Op::active(c, p, tg);
+    const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
+    const unsigned long long sbits = slice_pattern(S) << s;
+    const double s_thr = c.g.s_thr;
     typename Op::Acc acc;
     Op::init(acc);
-    const double s_thr = c.g.s_thr;
+    unsigned npairs = 0;
+    int chunk = 0;
     for (int row = 0; row < ROWS; ++row) {
         int j0, j1;
-        if (!row_range<ROWS>(c, cell, row, j0, j1)) continue;
-        for (int b = j0; b < j1; b += 64) {
+        if (!row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1)) continue;
+        for (int b = j0; b < j1; b += 64, ++chunk) {
             const int nb = min(64, j1 - b);
             if (lane < nb) Op::stage(c, b + lane, lane, lds);
             __syncthreads();
-            for (int j = 0; j < nb; j += S) {
-                const int jj = j + s;
-                if (jj < nb) {
-                    const double dx = tg.x - lds.x[jj];
-                    const double dy = tg.y - lds.y[jj];
-                    const double dz = FLAT ? 0.0 : tg.z - lds.z[jj];
-                    double ss = dx * dx + dy * dy;
-                    if (!FLAT) ss = ss + dz * dz;
-                    const int q = b + jj;
-                    if (act && ss <= s_thr && q != p)
-                        Op::interact(c, tg, lds, jj, p, q, dx, dy, dz, ss, acc);
+            unsigned long long m = 0ull;
+            if (load_masks) {
+                if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
+            } else {
+                for (int j = 0; j < nb; j += S) {
+                    const int jj = j + s;
+                    if (jj < nb) {
+                        const double2 xy = lds.xy[jj];
+                        const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                        double ss = dx * dx + dy * dy;
+                        if (!FLAT) {
+                            const double dz = tg.z - lds.zr[jj].x;
+                            ss = ss + dz * dz;
+                        }
+                        if (act && ss <= s_thr && b + jj != p) m |= 1ull << jj;
+                    }
+                }
+                if (store_masks) {
+                    unsigned long long full = m;
+                    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
+                    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
+                }
+            }
+            npairs += (unsigned)__popcll(m);
+            if (Op::INTERACT) {
+                while (__any(m != 0ull)) {
+                    if (m) {
+                        const int jj = __ffsll((long long)m) - 1;
+                        m &= m - 1ull;
+                        const double2 xy = lds.xy[jj];
+                        const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                        const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
+                        double ss = dx * dx + dy * dy;
+                        if (!FLAT) ss = ss + dz * dz;
+                        Op::interact(c, tg, lds, jj, p, b + jj, dx, dy, dz, ss, acc);
+                    }
                 }
             }
             __syncthreads();
         }
     }
-    if (c.pair_counter) count_pairs(c, Op::pairs(acc));
-    Op::reduce(acc, T);
-    if (act && s == 0) Op::store(c, p, tg, acc);
+    if (c.pair_counter) count_pairs(c, npairs);
+    Op::finish(c, p, tg, acc, T, act && s == 0);
 }
 
 template <int ROWS, bool FLAT, class Op>
 __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 {
-    __shared__ typename Op::Lds lds;
+    __shared__ Lds64 lds;
     const int ntiles = c.meta[META_NTILES];
+    const bool ok = c.masks != nullptr && c.meta[META_MASK_OK] != 0;
+    const bool load_masks = ok && c.mask_mode == MASK_LOAD;
+    const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
-        tile64<ROWS, FLAT, Op>(c, tile, lds);
+        tile64<ROWS, FLAT, Op>(c, tile, load_masks, store_masks, lds);
 }
 
 // =============================================================== mixed ====
@@ -409,15 +407,13 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 // partial sums are plain FP32 within a 64-candidate chunk and folded into a
 // compensated (sum, error) pair after every chunk.
 
-struct LdsPos32 {
-    float x[64], y[64], z[64];
-};
-struct LdsForce32 {
-    float x[64], y[64], z[64], vx[64], vy[64], vz[64], pr[64], rho[64];
+// LDS staging (FP32): (x, y, z, P/rho^2), (vx, vy, vz, rho).
+struct Lds32 {
+    float4 a[64], v[64];
 };
 
 struct Density32 {
-    using Lds = LdsPos32;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         float x, y, z;
     };
@@ -426,7 +422,6 @@ struct Density32 {
     };
     struct Acc {
         float s, e;
-        unsigned n;
     };
     static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
     {
@@ -434,29 +429,27 @@ struct Density32 {
         t.y = (float)(COL(c, C_Y, p) - oy);
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ bool active(const Tgt&) { return true; }
     static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds& s)
+                                 Lds32& s)
     {
-        s.x[l] = (float)(COL(c, C_X, q) - ox);
-        s.y[l] = (float)(COL(c, C_Y, q) - oy);
-        s.z[l] = (float)(COL(c, C_Z, q) - oz);
+        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
+                             (float)(COL(c, C_Z, q) - oz), 0.0f);
     }
-    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; a.n = 0; }
+    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int, int,
-                                    float, float, float, float ss, Part& q, Acc& a)
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds32&, int, int, int,
+                                    float, float, float, float ss, Part& q)
     {
         const float r = __builtin_sqrtf(ss);
         const float x = r * (float)c.g.inv_h;
         const float u = 1.0f - x;
         const float u2 = u * u;
         q.w = fmaf(u2 * u2, fmaf(4.0f, x, 1.0f), q.w);
-        a.n += 1;
     }
     static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.w); }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
-                                 bool write, int out_col)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
+                                  bool write, int out_col)
     {
         double w = (double)a.s + (double)a.e;
         for (int o = T; o < 64; o <<= 1) w += __shfl_xor(w, o);
@@ -466,11 +459,10 @@ struct Density32 {
         if (c.self) v = v + c.ph.kW;
         COL(c, out_col, p) = v;
     }
-    static __device__ unsigned pairs(const Acc& a) { return a.n; }
 };
 
 struct Force32 {
-    using Lds = LdsForce32;
+    static constexpr bool INTERACT = true;
     struct Tgt {
         float x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -480,7 +472,6 @@ struct Force32 {
     };
     struct Acc {
         float sx, sy, sz, ex, ey, ez, lx, ly, lz, fx, fy, fz;
-        unsigned n;
     };
     static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
     {
@@ -492,38 +483,34 @@ struct Force32 {
         t.rho = (float)COL(c, C_RHO, p);
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt& t) { return t.fluid; }
+    static __device__ bool active(const Tgt& t) { return t.fluid; }
     static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds& s)
+                                 Lds32& s)
     {
-        s.x[l] = (float)(COL(c, C_X, q) - ox);
-        s.y[l] = (float)(COL(c, C_Y, q) - oy);
-        s.z[l] = (float)(COL(c, C_Z, q) - oz);
-        s.vx[l] = (float)COL(c, C_VX, q); s.vy[l] = (float)COL(c, C_VY, q);
-        s.vz[l] = (float)COL(c, C_VZ, q);
-        s.pr[l] = (float)c.prho2[q];
-        s.rho[l] = (float)COL(c, C_RHO, q);
+        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
+                             (float)(COL(c, C_Z, q) - oz), (float)c.prho2[q]);
+        s.v[l] = make_float4((float)COL(c, C_VX, q), (float)COL(c, C_VY, q),
+                             (float)COL(c, C_VZ, q), (float)COL(c, C_RHO, q));
     }
     static __device__ void init(Acc& a)
     {
         a.sx = a.sy = a.sz = a.ex = a.ey = a.ez = 0.0f;
         a.lx = a.ly = a.lz = a.fx = a.fy = a.fz = 0.0f;
-        a.n = 0;
     }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds& s, int jj, int, int,
-                                    float dx, float dy, float dz, float ss, Part& q, Acc& a)
+    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds32& s, int jj, int, int,
+                                    float dx, float dy, float dz, float ss, Part& q)
     {
-        a.n += 1;
         const float r = __builtin_sqrtf(ss);
         const float u = 1.0f - r * (float)c.g.inv_h;
         const float u3 = u * u * u;
-        float A = t.pr + s.pr[jj];
-        const float dvx = t.vx - s.vx[jj], dvy = t.vy - s.vy[jj], dvz = t.vz - s.vz[jj];
+        const float4 a = s.a[jj], v = s.v[jj];
+        float A = t.pr + a.w;
+        const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
         if (c.ph.avk != 0.0) {
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
             if (vx < 0.0f) {
-                const float rhobar = 0.5f * (t.rho + s.rho[jj]);
+                const float rhobar = 0.5f * (t.rho + v.w);
                 const float den = rhobar * (ss + (float)c.ph.eps);
                 A = fmaf((float)c.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
             }
@@ -543,8 +530,8 @@ struct Force32 {
         two_sum_add(a.ly, a.fy, q.ly);
         two_sum_add(a.lz, a.fz, q.lz);
     }
-    static __device__ void store(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
-                                 bool write, int)
+    static __device__ void finish(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
+                                  bool write, int)
     {
         double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
                az = (double)a.sz + (double)a.ez;
@@ -564,12 +551,11 @@ struct Force32 {
         if (c.ph.mu != 0.0) { ox = ox + kl * lx; oy = oy + kl * ly; oz = oz + kl * lz; }
         COL(c, C_AX, p) = ox; COL(c, C_AY, p) = oy; COL(c, C_AZ, p) = oz;
     }
-    static __device__ unsigned pairs(const Acc& a) { return a.n; }
 };
 
 template <bool EMIT>
 struct Pairs32 {
-    using Lds = LdsPos32;
+    static constexpr bool INTERACT = EMIT;
     struct Tgt {
         float x, y, z;
     };
@@ -577,7 +563,7 @@ struct Pairs32 {
         int dummy;
     };
     struct Acc {
-        unsigned n;
+        int dummy;
     };
     static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
     {
@@ -585,39 +571,35 @@ struct Pairs32 {
         t.y = (float)(COL(c, C_Y, p) - oy);
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
-    static __device__ bool active(const PairCtx&, int, const Tgt&) { return true; }
+    static __device__ bool active(const Tgt&) { return true; }
     static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds& s)
+                                 Lds32& s)
     {
-        s.x[l] = (float)(COL(c, C_X, q) - ox);
-        s.y[l] = (float)(COL(c, C_Y, q) - oy);
-        s.z[l] = (float)(COL(c, C_Z, q) - oz);
+        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
+                             (float)(COL(c, C_Z, q) - oz), 0.0f);
     }
-    static __device__ void init(Acc& a) { a.n = 0; }
+    static __device__ void init(Acc&) {}
     static __device__ void zero(Part&) {}
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds&, int, int p, int q,
-                                    float, float, float, float, Part&, Acc& a)
+    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds32&, int, int p, int q,
+                                    float, float, float, float, Part&)
     {
-        if (EMIT) {
-            const unsigned long long k = atomicAdd(c.pair_counter, 1ull);
-            if ((long long)k < c.pair_cap) {
-                c.pair_out[2 * k] = c.refidx[p];
-                c.pair_out[2 * k + 1] = c.refidx[q];
-            }
-        } else {
-            a.n += 1;
+        const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
+        if ((long long)k < c.pair_cap) {
+            c.pair_out[2 * k] = c.refidx[p];
+            c.pair_out[2 * k + 1] = c.refidx[q];
         }
     }
     static __device__ void fold(Acc&, const Part&) {}
-    static __device__ void store(const PairCtx&, int, const Tgt&, const Acc&, int, bool, int) {}
-    static __device__ unsigned pairs(const Acc& a) { return EMIT ? 0u : a.n; }
+    static __device__ void finish(const PairCtx&, int, const Tgt&, const Acc&, int, bool, int) {}
 };
 
 template <int ROWS, bool FLAT, class Op>
-__device__ inline void tile32(const PairCtx& c, int tile, int out_col, typename Op::Lds& lds)
+__device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
+                              bool store_masks, Lds32& lds)
 {
-    const int2 td = c.tiles[tile];
+    const int4 td = c.tiles[tile];
     const int cell = td.x, first = td.y;
+    const long long mbase = mask_base(td);
     const int cend = c.cell_start[cell + 1];
     const int lane = threadIdx.x;
     const int T = c.T, S = 64 / T;
@@ -628,64 +610,94 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, typename 
     const double ox = COL(c, C_X, first), oy = COL(c, C_Y, first), oz = COL(c, C_Z, first);
     typename Op::Tgt tg;
     Op::load(c, pp, ox, oy, oz, tg);
-    const bool act = valid && Op::active(c, p, tg);
+    const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
     const double s_thr = c.g.s_thr;
     const float thr_lo = (float)(s_thr * (1.0 - BAND));
     const float thr_hi = (float)(s_thr * (1.0 + BAND));
+    const unsigned long long sbits = slice_pattern(S) << s;
     typename Op::Acc acc;
     Op::init(acc);
+    unsigned npairs = 0;
+    int chunk = 0;
     for (int row = 0; row < ROWS; ++row) {
         int j0, j1;
-        if (!row_range<ROWS>(c, cell, row, j0, j1)) continue;
-        for (int b = j0; b < j1; b += 64) {
+        if (!row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1)) continue;
+        for (int b = j0; b < j1; b += 64, ++chunk) {
             const int nb = min(64, j1 - b);
             if (lane < nb) Op::stage(c, b + lane, lane, ox, oy, oz, lds);
             __syncthreads();
-            typename Op::Part part;
-            Op::zero(part);
-            for (int j = 0; j < nb; j += S) {
-                const int jj = j + s;
-                const bool in = jj < nb;
-                const int jr = in ? jj : 0;
-                const float dx = tg.x - lds.x[jr];
-                const float dy = tg.y - lds.y[jr];
-                const float dz = FLAT ? 0.0f : tg.z - lds.z[jr];
-                const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                const int q = b + jr;
-                bool ok = ss <= thr_lo;
-                const bool band = in && act && ss > thr_lo && ss <= thr_hi;
-                if (__any(band)) {
-                    if (band) {
-                        // exact FP64 predicate of the reference
-                        const double e1 = px - COL(c, C_X, q), e2 = py - COL(c, C_Y, q);
-                        double es = e1 * e1 + e2 * e2;
-                        if (!FLAT) {
-                            const double e3 = pz - COL(c, C_Z, q);
-                            es = es + e3 * e3;
+            unsigned long long m = 0ull;
+            if (load_masks) {
+                if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
+            } else {
+                for (int j = 0; j < nb; j += S) {
+                    const int jj = j + s;
+                    const bool in = jj < nb;
+                    const int jr = in ? jj : 0;
+                    const float4 a = lds.a[jr];
+                    const float dx = tg.x - a.x, dy = tg.y - a.y;
+                    const float dz = FLAT ? 0.0f : tg.z - a.z;
+                    const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const int q = b + jr;
+                    bool ok = ss <= thr_lo;
+                    const bool band = in && act && ss > thr_lo && ss <= thr_hi;
+                    if (__any(band)) {
+                        if (band) {
+                            // exact FP64 predicate of the reference
+                            const double e1 = px - COL(c, C_X, q), e2 = py - COL(c, C_Y, q);
+                            double es = e1 * e1 + e2 * e2;
+                            if (!FLAT) {
+                                const double e3 = pz - COL(c, C_Z, q);
+                                es = es + e3 * e3;
+                            }
+                            ok = es <= s_thr;
                         }
-                        ok = es <= s_thr;
+                    }
+                    if (in && act && ok && q != p) m |= 1ull << jj;
+                }
+                if (store_masks) {
+                    unsigned long long full = m;
+                    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
+                    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
+                }
+            }
+            npairs += (unsigned)__popcll(m);
+            if (Op::INTERACT) {
+                typename Op::Part part;
+                Op::zero(part);
+                while (__any(m != 0ull)) {
+                    if (m) {
+                        const int jj = __ffsll((long long)m) - 1;
+                        m &= m - 1ull;
+                        const float4 a = lds.a[jj];
+                        const float dx = tg.x - a.x, dy = tg.y - a.y;
+                        const float dz = FLAT ? 0.0f : tg.z - a.z;
+                        const float ss =
+                            FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        Op::interact(c, tg, lds, jj, p, b + jj, dx, dy, dz, ss, part);
                     }
                 }
-                if (in && act && ok && q != p)
-                    Op::interact(c, tg, lds, jr, p, q, dx, dy, dz, ss, part, acc);
+                Op::fold(acc, part);
             }
-            Op::fold(acc, part);
             __syncthreads();
         }
     }
-    if (c.pair_counter) count_pairs(c, Op::pairs(acc));
-    Op::store(c, p, tg, acc, T, act && s == 0, out_col);
+    if (c.pair_counter) count_pairs(c, npairs);
+    Op::finish(c, p, tg, acc, T, act && s == 0, out_col);
 }
 
 template <int ROWS, bool FLAT, class Op>
 __global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
 {
-    __shared__ typename Op::Lds lds;
+    __shared__ Lds32 lds;
     const int ntiles = c.meta[META_NTILES];
+    const bool ok = c.masks != nullptr && c.meta[META_MASK_OK] != 0;
+    const bool load_masks = ok && c.mask_mode == MASK_LOAD;
+    const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
-        tile32<ROWS, FLAT, Op>(c, tile, out_col, lds);
+        tile32<ROWS, FLAT, Op>(c, tile, out_col, load_masks, store_masks, lds);
 }
 
 // ========================================================= diagnostics ====
@@ -703,7 +715,8 @@ __global__ __launch_bounds__(256) void k_candidates(PairCtx c, const int* __rest
         const int cell = key[p];
         for (int row = 0; row < ROWS; ++row) {
             int j0, j1;
-            if (row_range<ROWS>(c, cell, row, j0, j1)) all += (unsigned long long)(j1 - j0);
+            if (row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1))
+                all += (unsigned long long)(j1 - j0);
         }
         if (COL(c, C_TYPE, p) == 0.0) fluid = all;
     }

@@ -31,6 +31,7 @@ enum Meta {
     META_NTILES = 3,  // tiles of the pair kernels
     META_NCELLS = 4,  // non-empty cells
     META_QUEUE = 5,   // work-queue head of the pair kernels
+    META_MASK_OK = 6, // the pair-mask buffer holds this cell list's masks
     META_SLOTS = 8
 };
 
@@ -59,12 +60,20 @@ struct Phys {
     double eps;   // 0.01 * hs^2
 };
 
+// Pair-mask modes of the pair kernels.  A mask is the 64-bit set of
+// accepted candidates of one target in one 64-candidate chunk; the density
+// pass records them and the passes after it on the same cell list read
+// them instead of re-testing the candidates.
+enum MaskMode { MASK_TEST = 0, MASK_TEST_STORE = 1, MASK_LOAD = 2 };
+
 struct PairCtx {
     Geo g;
     Phys ph;
     int T;                    // targets per tile (8..64, power of two)
     int cap;
-    const int2* tiles;
+    const int4* tiles;        // {cell, first target, mask base lo, mask base hi}
+    unsigned long long* masks;
+    int mask_mode;
     const int* cell_start;
     const int* meta;
     int* queue;               // &meta[META_QUEUE]
@@ -72,6 +81,7 @@ struct PairCtx {
     const int* refidx;
     const double* prho2;      // P/rho^2 per particle (force pass)
     unsigned long long* pair_counter;
+    unsigned long long* emit_counter;
     long long* pair_out;      // pair emission (test hook)
     long long pair_cap;
     int self;                 // add the self term (density)
@@ -89,6 +99,26 @@ enum PairOp {
     P_EMIT = 6,          // pair emission
     NPAIROP
 };
+
+// Row r of the 3 (2D) or 9 (3D) stencil rows of `cell`: the cells
+// key + L1*dj + L1*L2*dk + di for di = -1..1 (src/structs.jl:70-82) are
+// consecutive, so a row is one contiguous particle range [j0, j1) after the
+// linear bounds test 1 <= key <= key_max of src/core.jl:98.
+template <int ROWS>
+__device__ inline bool row_range(const Geo& g, const int* cell_start, int cell, int row, int& j0,
+                                 int& j1)
+{
+    const int dj = row % 3 - 1;
+    const int dk = ROWS == 3 ? 0 : row / 3 - 1;
+    const long long mid = (long long)cell + (long long)g.L1 * dj + (long long)g.L1 * g.L2 * dk;
+    long long lo = mid - 1, hi = mid + 1;
+    if (lo < 0) lo = 0;
+    if (hi > g.key_max - 1) hi = g.key_max - 1;
+    if (lo > hi) return false;
+    j0 = cell_start[lo];
+    j1 = cell_start[hi + 1];
+    return j0 < j1;
+}
 
 // Kernel launchers (implemented in the .hip files).
 void launch_pair(int op, int precision, const PairCtx& c, int grid, hipStream_t st);
@@ -128,8 +158,16 @@ struct sph_system {
     int* tmp_ref;        // cap
     int* perm;           // cap
     int* inv;            // cap
-    int2* tiles;         // tile bound
+    int4* tiles;         // tile bound
     long long tile_cap;
+    int* cell_chunks;    // key_max: 64-candidate chunks of a target of the cell
+    unsigned long long* mask_start;  // key_max + 2: first mask word of each cell
+    unsigned long long* scan_part64;
+    unsigned long long* masks;       // pair masks of the current cell list
+    long long mask_cap;              // words allocated
+    unsigned long long* mask_total;  // device: words needed (1 value)
+    bool masks_valid;                // masks match the cell list and positions
+    bool use_masks;                  // option: reuse masks across passes
     double* prho2;       // cap
     double* stage;       // 3 * cap (field IO)
     double* red_part;    // reduction partials

@@ -211,6 +211,11 @@ class ParticleSystem:
         check(lib().sph_autotune(self._h, ctypes.byref(p), ctypes.byref(best)))
         return best.value
 
+    def set_pair_masks(self, enable):
+        """Reuse the density pass's accepted-candidate masks in later pair
+        passes on the same cell list (default on)."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_PAIR_MASKS, 1 if enable else 0))
+
     def step_pairs(self):
         """(density-pass pairs, force-pass pairs) of the last step() call."""
         d = ctypes.c_int64()

@@ -46,6 +46,8 @@ SPH_REDUCE_ENERGY_RHO0 = 3
 SPH_SCHEME_WCSPH_SE = 0
 SPH_SCHEME_WCSPH_SE_RHO0 = 1
 
+SPH_OPTION_PAIR_MASKS = 0
+
 SPH_KERNEL_BUILD = 0
 SPH_KERNEL_DENSITY = 1
 SPH_KERNEL_FORCE = 2
@@ -94,6 +96,7 @@ SIGNATURES = {
     "sph_set_tile": (_i, [_p, _i]),
     "sph_autotune": (_i, [_p, ctypes.POINTER(sph_params), ctypes.POINTER(_i)]),
     "sph_candidates": (_i, [_p, _pi64, _pi64]),
+    "sph_set_option": (_i, [_p, _i, _i]),
     "sph_profile": (_i, [_p, _i]),
     "sph_profile_read": (_i, [_p, _i, _pd, _pi64]),
     "sph_last_error": (ctypes.c_char_p, []),

@@ -290,3 +290,27 @@ def test_autotune_keeps_state():
     assert T in (8, 16, 32, 64)
     for f, a in before.items():
         assert np.array_equal(g.get(f), a)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pair_masks_match_retesting(precision, dim):
+    """Forces read from the density pass's accepted-candidate masks equal
+    the forces of a pass that re-tests every candidate, bit for bit."""
+    n = 3000 if dim == 2 else 6000
+    x, lo, hi, rng = random_cloud(n, dim, 91 + dim)
+    h = 0.06 if dim == 2 else 0.15
+    v, rho, P, typ = fields(rng, n, dim)
+    out = []
+    for masks in (True, False):
+        g, _ = make_both(x, h, lo, hi, precision, v, rho, P, typ)
+        g.set_pair_masks(masks)
+        sp.create_cell_list(g)
+        sp.apply(g, sp.reset_rho)
+        sp.apply(g, sp.find_rho, self=True, m=0.37)
+        sp.apply(g, sp.find_pressure, c=50.0, rho0=1000.0)
+        sp.apply(g, sp.reset_a)
+        sp.apply(g, sp.internal_force, **dict(PHYS, hs=h / 2))
+        out.append((g.get("rho"), g.get("a")))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
