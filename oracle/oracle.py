@@ -38,6 +38,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         sig = {
             "orc_create": (ctypes.c_void_p, [ctypes.c_double, _pd, _pd]),
+            "orc_create_periodic": (ctypes.c_void_p, [ctypes.c_double, _pd, _pd]),
             "orc_destroy": (None, [ctypes.c_void_p]),
             "orc_resize": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
             "orc_n": (ctypes.c_int64, [ctypes.c_void_p]),
@@ -93,12 +94,16 @@ def params(**kw):
 
 
 class OracleSystem:
-    """The reference's ParticleSystem restated in C (FP64, CPU)."""
+    """The reference's ParticleSystem restated in C (FP64, CPU).  With
+    periodic=True, a system periodic along every axis of the box with extent
+    (not in the reference)."""
 
-    def __init__(self, h, lo, hi):
+    def __init__(self, h, lo, hi, periodic=False):
         lo = np.ascontiguousarray(lo, dtype=np.float64)
         hi = np.ascontiguousarray(hi, dtype=np.float64)
-        self._s = lib().orc_create(float(h), lo.ctypes.data_as(_pd), hi.ctypes.data_as(_pd))
+        make = lib().orc_create_periodic if periodic else lib().orc_create
+        self._s = make(float(h), lo.ctypes.data_as(_pd), hi.ctypes.data_as(_pd))
+        self.periodic = periodic
         if not self._s:
             raise ValueError("invalid oracle system")
         self.h = h

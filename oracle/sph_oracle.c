@@ -30,6 +30,8 @@
  *   integrator ............... free-fall closed form
  *   energy ................... closed-form energy of hand-built states and the
  *                              energy-drift bound of tests/test_collision_2d.jl
+ *   periodic systems ......... numpy minimum-image brute force
+ *                              (tests/test_oracle_periodic.py)
  * Parity unpinned at the ulp level: the @fastmath kernels (src/kernels.jl)
  * and Julia's libm log in the energy; those are compared by tolerance.
  */
@@ -70,6 +72,10 @@ typedef struct orc_sys {
     int64_t *cell_start;                     /* [key_max+1], 0-based cells */
     int64_t *cell_idx;                       /* [cap], descending per cell */
     int cells_valid;
+    /* periodic systems (not in the reference): minimum-image pairs in a box
+     * periodic along x, y (and z unless bmin[2] == bmax[2]) */
+    int periodic, axes;
+    double box[3], cell[3];
 } orc_sys;
 
 /* ---------------------------------------------------------------- keys -- */
@@ -126,6 +132,70 @@ static int is_inside_box(const orc_sys* s, const double* x)
     return s->bmin[0] <= x[0] && x[0] <= s->bmax[0] &&
            s->bmin[1] <= x[1] && x[1] <= s->bmax[1] &&
            s->bmin[2] <= x[2] && x[2] <= s->bmax[2];
+}
+
+/* Periodic systems: cell of a position wrapped into the box,
+ * floor((x - bmin) / cell) clamped to [0, L-1] on each periodic axis. */
+static void periodic_cell(const orc_sys* s, const double* x, int64_t* c)
+{
+    for (int d = 0; d < 3; ++d) {
+        c[d] = 0;
+        if (d >= s->axes) continue;
+        int64_t v = (int64_t)floor((x[d] - s->bmin[d]) / s->cell[d]);
+        c[d] = v < 0 ? 0 : (v >= s->key_lim[d] ? s->key_lim[d] - 1 : v);
+    }
+}
+
+/* x - box * floor((x - bmin) / box) on each periodic axis. */
+static void periodic_wrap(const orc_sys* s, double* x)
+{
+    for (int d = 0; d < s->axes; ++d)
+        x[d] = x[d] - s->box[d] * floor((x[d] - s->bmin[d]) / s->box[d]);
+}
+
+static int is_finite3(const double* x)
+{
+    return isfinite(x[0]) && isfinite(x[1]) && isfinite(x[2]);
+}
+
+/* Cells particle p visits (0-based) with the image shift added to the
+ * candidates' coordinates.  Reference systems: key + key_diff with the
+ * linear bounds test of src/core.jl:96-99, no shift.  Periodic systems: the
+ * 3^dim neighbouring cells, wrapped per axis, shifted by -+box across the
+ * seam (every periodic axis has L >= 3 cells of size >= h, so each
+ * neighbouring cell appears once and holds the minimum image). */
+static int neighbour_cells(const orc_sys* s, const double* xp, int64_t* cells,
+                           double (*shift)[3])
+{
+    int n = 0;
+    if (!s->periodic) {
+        int64_t key = find_key(s, xp);
+        for (int t = 0; t < s->n_diff; ++t) {
+            int64_t nk = key + s->key_diff[t];
+            if (1 <= nk && nk <= s->key_max) {
+                cells[n] = nk - 1;
+                shift[n][0] = shift[n][1] = shift[n][2] = 0.0;
+                ++n;
+            }
+        }
+        return n;
+    }
+    int64_t c[3];
+    periodic_cell(s, xp, c);
+    int kr = s->axes == 3 ? 1 : 0;
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -kr; dk <= kr; ++dk) {
+                int64_t idx[3] = {c[0] + di, c[1] + dj, c[2] + dk};
+                for (int d = 0; d < 3; ++d) {
+                    shift[n][d] = 0.0;
+                    if (idx[d] < 0) { idx[d] += s->key_lim[d]; shift[n][d] = -s->box[d]; }
+                    else if (idx[d] >= s->key_lim[d]) { idx[d] -= s->key_lim[d]; shift[n][d] = s->box[d]; }
+                }
+                cells[n] = idx[0] + s->key_lim[0] * (idx[1] + s->key_lim[1] * idx[2]);
+                ++n;
+            }
+    return n;
 }
 
 /* src/core.jl:8-10, src/algebra.jl:49-60 -- FP64 distance, no FMA. */
@@ -219,6 +289,42 @@ orc_sys* orc_create(double h, const double* bmin, const double* bmax)
     return s;
 }
 
+/* A periodic system (not in the reference) on the box [bmin, bmax): each
+ * periodic axis is split into L = floor(box/h) cells (reduced while
+ * box/L < h), L >= 3.  Returns NULL when the box is too small. */
+orc_sys* orc_create_periodic(double h, const double* bmin, const double* bmax)
+{
+    if (!(h > 0.0)) return NULL;
+    orc_sys* s = (orc_sys*)calloc(1, sizeof(orc_sys));
+    s->h = h;
+    memcpy(s->bmin, bmin, 3 * sizeof(double));
+    memcpy(s->bmax, bmax, 3 * sizeof(double));
+    s->periodic = 1;
+    s->axes = bmin[2] == bmax[2] ? 2 : 3;
+    for (int d = 0; d < 3; ++d) {
+        s->key_phase[d] = 0;
+        if (d >= s->axes) {
+            s->key_lim[d] = 1;
+            s->cell[d] = 1.0;
+            continue;
+        }
+        s->box[d] = bmax[d] - bmin[d];
+        double L = floor(s->box[d] / h);
+        while (L >= 1.0 && s->box[d] / L < h) L -= 1.0;
+        if (!(L >= 3.0)) {
+            free(s);
+            return NULL;
+        }
+        s->key_lim[d] = (int64_t)L;
+        s->cell[d] = s->box[d] / L;
+    }
+    s->key_max = s->key_lim[0] * s->key_lim[1] * s->key_lim[2];
+    s->dim = s->key_lim[2] == 1 ? 2 : 3;
+    s->n_diff = s->axes == 3 ? 27 : 9;
+    s->cell_start = (int64_t*)calloc((size_t)s->key_max + 1, sizeof(int64_t));
+    return s;
+}
+
 void orc_destroy(orc_sys* s)
 {
     if (!s) return;
@@ -304,10 +410,17 @@ static void copy_particle(orc_sys* s, int64_t dst, int64_t src)
 int64_t orc_create_cell_list(orc_sys* s)
 {
     int64_t n = s->n;
+    /* periodic systems wrap every finite position into the box and
+     * remove only non-finite ones */
+    if (s->periodic)
+        for (int64_t i = 0; i < n; ++i)
+            if (is_finite3(s->x + 3 * i)) periodic_wrap(s, s->x + 3 * i);
     /* removal: collect indices outside the box in descending order */
     int64_t m = 0;
-    for (int64_t i = n - 1; i >= 0; --i)
-        if (!is_inside_box(s, s->x + 3 * i)) s->cell_idx[m++] = i;
+    for (int64_t i = n - 1; i >= 0; --i) {
+        int keep = s->periodic ? is_finite3(s->x + 3 * i) : is_inside_box(s, s->x + 3 * i);
+        if (!keep) s->cell_idx[m++] = i;
+    }
     for (int64_t i = 0; i < m; ++i)   /* particles[r_i] = particles[end+1-i] */
         copy_particle(s, s->cell_idx[i], n - 1 - i);
     n -= m;
@@ -318,7 +431,13 @@ int64_t orc_create_cell_list(orc_sys* s)
     memset(s->cell_start, 0, (size_t)(km + 1) * sizeof(int64_t));
     int64_t* key = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     for (int64_t i = 0; i < n; ++i) {
-        key[i] = find_key(s, s->x + 3 * i) - 1;   /* 0-based */
+        if (s->periodic) {
+            int64_t c[3];
+            periodic_cell(s, s->x + 3 * i, c);
+            key[i] = c[0] + s->key_lim[0] * (c[1] + s->key_lim[1] * c[2]);
+        } else {
+            key[i] = find_key(s, s->x + 3 * i) - 1;   /* 0-based */
+        }
         s->cell_start[key[i] + 1]++;
     }
     for (int64_t c = 0; c < km; ++c) s->cell_start[c + 1] += s->cell_start[c];
@@ -332,21 +451,26 @@ int64_t orc_create_cell_list(orc_sys* s)
 }
 
 /* Visit every accepted ordered pair of particle p in reference order:
- * src/core.jl:94-112 (_apply_binary!). */
+ * src/core.jl:94-112 (_apply_binary!).  The body sees q, r and xq, the
+ * candidate's position with the image shift added (the position itself for
+ * reference systems). */
 #define FOR_EACH_NEIGHBOUR(s, p, ...)                                          \
     do {                                                                       \
-        int64_t key_ = find_key((s), (s)->x + 3 * (p));                        \
-        for (int t_ = 0; t_ < (s)->n_diff; ++t_) {                             \
-            int64_t nk_ = key_ + (s)->key_diff[t_];                            \
-            if (1 <= nk_ && nk_ <= (s)->key_max) {                             \
-                int64_t c_ = nk_ - 1;                                          \
-                for (int64_t e_ = (s)->cell_start[c_];                         \
-                     e_ < (s)->cell_start[c_ + 1]; ++e_) {                     \
-                    int64_t q = (s)->cell_idx[e_];                             \
-                    double r = dist3((s)->x + 3 * (p), (s)->x + 3 * q);        \
-                    if (r > (s)->h || (p) == q) continue;                      \
-                    __VA_ARGS__                                                \
-                }                                                              \
+        int64_t cells_[27];                                                    \
+        double sh_[27][3];                                                     \
+        const double* xp_ = (s)->x + 3 * (p);                                  \
+        int nc_ = neighbour_cells((s), xp_, cells_, sh_);                      \
+        for (int t_ = 0; t_ < nc_; ++t_) {                                     \
+            int64_t c_ = cells_[t_];                                           \
+            for (int64_t e_ = (s)->cell_start[c_];                             \
+                 e_ < (s)->cell_start[c_ + 1]; ++e_) {                         \
+                int64_t q = (s)->cell_idx[e_];                                 \
+                const double* xs_ = (s)->x + 3 * q;                            \
+                double xq[3] = {xs_[0] + sh_[t_][0], xs_[1] + sh_[t_][1],      \
+                                xs_[2] + sh_[t_][2]};                          \
+                double r = dist3(xp_, xq);                                     \
+                if (r > (s)->h || (p) == q) continue;                          \
+                __VA_ARGS__                                                    \
             }                                                                  \
         }                                                                      \
     } while (0)
@@ -374,7 +498,7 @@ int64_t orc_neighbour_counts(const orc_sys* s, int64_t* counts)
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : tot)
     for (int64_t p = 0; p < s->n; ++p) {
         int64_t c = 0;
-        FOR_EACH_NEIGHBOUR(s, p, { (void)q; (void)r; ++c; });
+        FOR_EACH_NEIGHBOUR(s, p, { (void)q; (void)r; (void)xq; ++c; });
         if (counts) counts[p] = c;
         tot += c;
     }
@@ -409,7 +533,7 @@ int64_t orc_density_sum(orc_sys* s, const orc_params* pr, int self)
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : cnt)
     for (int64_t p = 0; p < s->n; ++p) {
         double acc = s->rho[p];
-        FOR_EACH_NEIGHBOUR(s, p, { (void)q; acc = acc + m * W(s, r); ++cnt; });
+        FOR_EACH_NEIGHBOUR(s, p, { (void)q; (void)xq; acc = acc + m * W(s, r); ++cnt; });
         if (self) acc = acc + m * W(s, 0.0);
         s->rho[p] = acc;
     }
@@ -424,7 +548,7 @@ void orc_density_sum_rho0(orc_sys* s, const orc_params* pr, int self)
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t p = 0; p < s->n; ++p) {
         double acc = s->rho0[p];
-        FOR_EACH_NEIGHBOUR(s, p, { (void)q; acc = acc + m * W(s, r); });
+        FOR_EACH_NEIGHBOUR(s, p, { (void)q; (void)xq; acc = acc + m * W(s, r); });
         if (self) acc = acc + m * W(s, 0.0);
         s->rho0[p] = acc;
     }
@@ -447,7 +571,6 @@ void orc_internal_force(orc_sys* s, const orc_params* pr)
         double Ap = s->P[p] / (s->rho[p] * s->rho[p]);
         double a0 = s->a[3 * p], a1 = s->a[3 * p + 1], a2 = s->a[3 * p + 2];
         FOR_EACH_NEIGHBOUR(s, p, {
-            const double* xq = s->x + 3 * q;
             const double* vq = s->v + 3 * q;
             double dk = rDW(s, r);
             double ker = m * dk;
@@ -485,7 +608,6 @@ void orc_pressure_force_rho0(orc_sys* s, const orc_params* pr)
         const double* xp = s->x + 3 * p;
         double a0 = s->a[3 * p], a1 = s->a[3 * p + 1], a2 = s->a[3 * p + 2];
         FOR_EACH_NEIGHBOUR(s, p, {
-            const double* xq = s->x + 3 * q;
             double ker = m * rDW(s, r);
             double sp = -ker * (s->P[p] / r02 + s->P[q] / r02);
             a0 = a0 + sp * (xp[0] - xq[0]);
@@ -507,7 +629,6 @@ void orc_balance_of_mass(orc_sys* s, const orc_params* pr)
         const double* vp = s->v + 3 * p;
         double acc = s->Drho[p];
         FOR_EACH_NEIGHBOUR(s, p, {
-            const double* xq = s->x + 3 * q;
             const double* vq = s->v + 3 * q;
             double ker = m * rDW(s, r);
             double d = (xp[0] - xq[0]) * (vp[0] - vq[0]) +

@@ -1,10 +1,17 @@
-// C ABI of libsph_hip (include/sph.h): system lifetime, device memory,
-// operator dispatch by registered id, synchronisation and error reporting.
+// C ABI of libsph_hip (include/sph.h, include/sph_batch.h): handle lifetime,
+// device memory, the systems of a batch, operator dispatch by registered id,
+// synchronisation and error reporting.
+//
+// A handle is a batch of systems.  The single-system functions of sph.h act
+// on slot 0 of a handle made by sph_create; sph_batch.h adds and retires
+// systems of a handle made by sph_batch_create.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
 
+#include "sph_batch.h"
 #include "sph_internal.h"
 
 using namespace sph;
@@ -12,14 +19,19 @@ using namespace sph;
 namespace sph {
 void enqueue_build(sph_system* s);
 void enqueue_tiles(sph_system* s);
-void enqueue_unary(sph_system* s, int op, const Phys& ph, double* prho2);
+void enqueue_unary(sph_system* s, int op, double* prho2);
 void enqueue_prho2(sph_system* s);
-void enqueue_kick_drift(sph_system* s, const Phys& ph);
-void enqueue_reduce(sph_system* s, int what, const Phys& ph, double* out_dev);
-void enqueue_field_out(sph_system* s, int col0, int ncomp, double* out_dev);
-void enqueue_field_in(sph_system* s, int col0, int ncomp, const double* in_dev);
-void enqueue_append(sph_system* s, int n0, int m, const double* in_dev, int has_v, int has_rho,
-                    int has_P, int has_type);
+void enqueue_kick_drift(sph_system* s);
+void enqueue_reduce(sph_system* s, int sys, long long off, long long n, int what, double* out_dev);
+void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,
+                       double* out_dev);
+void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,
+                      const double* in_dev);
+void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const double* in_dev,
+                    int has_v, int has_rho, int has_P, int has_type);
+void enqueue_set_int(sph_system* s, int* p, int v);
+void enqueue_fill_int(sph_system* s, int* p, int v, long long n);
+void enqueue_remap_sys(sph_system* s, const int* map_dev);
 long long scan_partials_needed(long long n);
 long long reduce_partials_needed(long long n);
 void reset_queue(sph_system* s);
@@ -52,8 +64,69 @@ double sqrt_threshold(double h)
     return s;
 }
 
+// Cell geometry of one system.  Reference systems follow src/structs.jl:63-82.
+// Periodic systems (not in the reference) split each periodic axis of the
+// box into L = floor(box/h) >= 3 cells of length box/L >= h, so the 3-cell
+// stencil holds every minimum-image neighbour exactly once.
+int make_geo(double h, const double* bmin, const double* bmax, int periodic, Geo* out)
+{
+    Geo g{};
+    if (!(h > 0.0) || !std::isfinite(h))
+        return fail(SPH_EINVAL, "invalid ParticleSystem declaration! (h must be a positive float)");
+    g.h = h;
+    g.inv_h = 1.0 / h;
+    g.s_thr = sqrt_threshold(h);
+    for (int d = 0; d < 3; ++d) {
+        if (!std::isfinite(bmin[d]) || !std::isfinite(bmax[d]) || bmin[d] > bmax[d])
+            return fail(SPH_EINVAL, "invalid bounding box");
+        g.bmin[d] = bmin[d];
+        g.bmax[d] = bmax[d];
+    }
+    g.flat = bmin[2] == bmax[2];
+    long long lim[3];
+    if (!periodic) {
+        for (int d = 0; d < 3; ++d) {
+            // src/structs.jl:66-67
+            const double fmin = std::floor(bmin[d] / h), fmax = std::floor(bmax[d] / h);
+            if (std::fabs(fmin) > 1e15 || std::fabs(fmax) > 1e15)
+                return fail(SPH_EINVAL, "bounding box too large for h");
+            g.kp[d] = (long long)fmin;
+            lim[d] = (long long)fmax - g.kp[d] + 1;
+        }
+    } else {
+        const int axes = g.flat ? 2 : 3;
+        for (int d = 0; d < 3; ++d) {
+            g.kp[d] = 0;
+            if (d >= axes) {
+                lim[d] = 1;
+                g.box[d] = 0.0;
+                g.cell[d] = 1.0;
+                continue;
+            }
+            const double box = bmax[d] - bmin[d];
+            double L = std::floor(box / h);
+            while (L >= 1.0 && box / L < h) L -= 1.0;
+            if (!(L >= 3.0) || L > 1e9)
+                return fail(SPH_EINVAL, "a periodic box needs at least 3 h along every periodic axis");
+            lim[d] = (long long)L;
+            g.box[d] = box;
+            g.cell[d] = box / L;
+        }
+        g.periodic = 1;
+    }
+    const long long km = lim[0] * lim[1] * lim[2];
+    if (km <= 0 || km > (1ll << 30)) return fail(SPH_EINVAL, "cell grid too large (key_max > 2^30)");
+    g.L1 = (int)lim[0];
+    g.L2 = (int)lim[1];
+    g.L3 = (int)lim[2];
+    g.key_max = (int)km;
+    g.dim = lim[2] == 1 ? 2 : 3;  // src/structs.jl:70
+    *out = g;
+    return 0;
+}
+
 // Wendland quintic constants of src/kernels.jl:108-195.
-Phys make_phys(const sph_system* s, const sph_params* p)
+Phys make_phys(const Geo& g, const sph_params* p)
 {
     Phys ph{};
     if (p) {
@@ -62,8 +135,8 @@ Phys make_phys(const sph_system* s, const sph_params* p)
         ph.g[0] = p->g[0]; ph.g[1] = p->g[1]; ph.g[2] = p->g[2];
         ph.dt = p->dt; ph.coeff = p->coeff;
     }
-    const double h = s->geo.h;
-    if (s->geo.dim == 2) {
+    const double h = g.h;
+    if (g.dim == 2) {
         ph.kW = ph.m * (2.228169203286535 / (h * h));
         ph.kdW = ph.m * (-44.563384065730695 / ((h * h) * (h * h)));
     } else {
@@ -75,32 +148,50 @@ Phys make_phys(const sph_system* s, const sph_params* p)
     return ph;
 }
 
+// Device allocation with the handle's byte count kept current.
 template <class T>
-int dalloc(T** p, long long n)
+int dalloc(sph_system* s, T** p, long long n)
 {
     *p = nullptr;
     if (n <= 0) n = 1;
     hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
-    if (e != hipSuccess) return fail(SPH_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SPH_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    s->bytes += (long long)sizeof(T) * n;
     return 0;
 }
 
 template <class T>
-void dfree(T*& p)
+void dfree(sph_system* s, T*& p, long long n)
 {
-    if (p) hipFree((void*)p);
+    if (p) {
+        (void)hipFree((void*)p);
+        s->bytes -= (long long)sizeof(T) * (n > 0 ? n : 1);
+    }
     p = nullptr;
 }
 
+int nsys(const sph_system* s) { return (int)s->sys_host.size(); }
+
 void free_particle_arrays(sph_system* s)
 {
+    const long long cap = s->cap;
     for (int b = 0; b < 2; ++b) {
-        dfree(s->cols[b]);
-        dfree(s->icols[b]);
+        dfree(s, s->cols[b], NCOL * cap);
+        dfree(s, s->icols[b], NICOL * cap);
     }
-    dfree(s->slot); dfree(s->tmp_idx); dfree(s->tmp_key); dfree(s->tmp_ref);
-    dfree(s->perm); dfree(s->inv); dfree(s->tiles); dfree(s->prho2); dfree(s->stage);
-    dfree(s->red_part);
+    dfree(s, s->slot, cap); dfree(s, s->tmp_idx, cap); dfree(s, s->tmp_key, cap);
+    dfree(s, s->tmp_ref, cap); dfree(s, s->perm, cap); dfree(s, s->inv, cap);
+    dfree(s, s->tiles, s->tile_cap); dfree(s, s->prho2, cap); dfree(s, s->stage, 9 * cap);
+    dfree(s, s->red_part, s->red_part_n);
+}
+
+// Tiles of a cell list: sum over cells of ceil(count/T) <= n/8 + non-empty cells.
+long long tile_bound(const sph_system* s)
+{
+    return s->cap / 8 + std::min<long long>(s->cap, std::max<long long>(s->cells_cap, 1)) + 2;
 }
 
 // Grow the particle arrays to hold `need` particles, keeping the contents
@@ -115,10 +206,16 @@ int ensure_capacity(sph_system* s, long long need)
     if (ncap >= (1ll << 31) - 64) ncap = need;
     double* nc[2] = {nullptr, nullptr};
     int* ni[2] = {nullptr, nullptr};
-    int rc;
-    for (int b = 0; b < 2; ++b) {
-        if ((rc = dalloc(&nc[b], NCOL * ncap))) return rc;
-        if ((rc = dalloc(&ni[b], NICOL * ncap))) return rc;
+    int rc = 0;
+    for (int b = 0; b < 2 && !rc; ++b) {
+        if (!(rc = dalloc(s, &nc[b], NCOL * ncap))) rc = dalloc(s, &ni[b], NICOL * ncap);
+    }
+    if (rc) {
+        for (int b = 0; b < 2; ++b) {
+            dfree(s, nc[b], NCOL * ncap);
+            dfree(s, ni[b], NICOL * ncap);
+        }
+        return rc;
     }
     if (s->cap > 0 && s->n > 0) {
         for (int f = 0; f < NCOL; ++f)
@@ -127,62 +224,154 @@ int ensure_capacity(sph_system* s, long long need)
         for (int f = 0; f < NICOL; ++f)
             HIPC(hipMemcpyAsync(ni[s->cur] + f * ncap, s->icols[s->cur] + f * s->cap,
                                 sizeof(int) * s->n, hipMemcpyDeviceToDevice, s->stream));
-        HIPC(hipStreamSynchronize(s->stream));
     }
+    HIPC(hipStreamSynchronize(s->stream));
     free_particle_arrays(s);
     s->cols[0] = nc[0]; s->cols[1] = nc[1];
     s->icols[0] = ni[0]; s->icols[1] = ni[1];
     s->cap = ncap;
-    s->tile_cap = ncap / 8 + std::min<long long>(ncap, s->geo.key_max) + 2;
+    s->tile_cap = tile_bound(s);
     s->red_part_n = 3 * reduce_partials_needed(ncap);
-    if ((rc = dalloc(&s->slot, ncap)) || (rc = dalloc(&s->tmp_idx, ncap)) ||
-        (rc = dalloc(&s->tmp_key, ncap)) || (rc = dalloc(&s->tmp_ref, ncap)) ||
-        (rc = dalloc(&s->perm, ncap)) || (rc = dalloc(&s->inv, ncap)) ||
-        (rc = dalloc(&s->tiles, s->tile_cap)) || (rc = dalloc(&s->prho2, ncap)) ||
-        (rc = dalloc(&s->stage, 9 * ncap)) || (rc = dalloc(&s->red_part, s->red_part_n)))
+    if ((rc = dalloc(s, &s->slot, ncap)) || (rc = dalloc(s, &s->tmp_idx, ncap)) ||
+        (rc = dalloc(s, &s->tmp_key, ncap)) || (rc = dalloc(s, &s->tmp_ref, ncap)) ||
+        (rc = dalloc(s, &s->perm, ncap)) || (rc = dalloc(s, &s->inv, ncap)) ||
+        (rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->prho2, ncap)) ||
+        (rc = dalloc(s, &s->stage, 9 * ncap)) || (rc = dalloc(s, &s->red_part, s->red_part_n)))
         return rc;
     return 0;
 }
 
+void free_cell_arrays(sph_system* s)
+{
+    const long long c = s->cells_cap;
+    dfree(s, s->cnt, c); dfree(s, s->cell_start, c + 1); dfree(s, s->tile_start, c + 1);
+    dfree(s, s->cell_chunks, c); dfree(s, s->mask_start, c + 1); dfree(s, s->cell_sys, c);
+    dfree(s, s->scan_part, s->scan_part_n); dfree(s, s->scan_part64, s->scan_part_n);
+}
+
+// Per-cell arrays for `buckets` buckets (the cells of every system, one
+// removal bucket per system and one bucket for retired systems); the
+// cell -> system map is refilled from the host table.
+int ensure_cells(sph_system* s, long long buckets)
+{
+    if (buckets <= s->cells_cap) return 0;
+    const long long ncap = std::max<long long>(buckets, s->cells_cap + s->cells_cap / 2);
+    if (ncap >= (1ll << 31) - 64) return fail(SPH_EINVAL, "too many cells in the batch");
+    free_cell_arrays(s);
+    s->cells_cap = ncap;
+    s->scan_part_n = scan_partials_needed(ncap + 2);
+    int rc;
+    if ((rc = dalloc(s, &s->cnt, ncap)) || (rc = dalloc(s, &s->cell_start, ncap + 1)) ||
+        (rc = dalloc(s, &s->tile_start, ncap + 1)) || (rc = dalloc(s, &s->cell_chunks, ncap)) ||
+        (rc = dalloc(s, &s->mask_start, ncap + 1)) || (rc = dalloc(s, &s->cell_sys, ncap)) ||
+        (rc = dalloc(s, &s->scan_part, s->scan_part_n)) ||
+        (rc = dalloc(s, &s->scan_part64, s->scan_part_n))) {
+        s->cells_cap = 0;
+        return rc;
+    }
+    if (s->cap > 0) {  // the tile bound grows with the cell count
+        dfree(s, s->tiles, s->tile_cap);
+        s->tile_cap = tile_bound(s);
+        if ((rc = dalloc(s, &s->tiles, s->tile_cap))) return rc;
+    }
+    for (int i = 0; i < nsys(s); ++i)
+        enqueue_fill_int(s, s->cell_sys + s->sys_host[i].cell_base, i, s->sys_host[i].g.key_max);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+// Copy the host table of systems to the device, growing the device table
+// (and the per-system counters, restarted from the host mirror) if needed.
+int upload_sysd(sph_system* s)
+{
+    const int n = nsys(s);
+    if (n > s->sysd_cap) {
+        const long long cap = std::max<long long>(16, 2 * (long long)n);
+        SysDesc* nd = nullptr;
+        int* nc = nullptr;
+        int* nh = nullptr;
+        int rc;
+        if ((rc = dalloc(s, &nd, cap))) return rc;
+        if ((rc = dalloc(s, &nc, SYSCNT * cap))) {
+            dfree(s, nd, cap);
+            return rc;
+        }
+        hipError_t e = hipHostMalloc((void**)&nh, sizeof(int) * SYSCNT * cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            dfree(s, nd, cap);
+            dfree(s, nc, SYSCNT * cap);
+            return fail(SPH_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        std::memset(nh, 0, sizeof(int) * SYSCNT * cap);
+        if (s->sys_cnt_cap > 0) {
+            // keep the device counters of the existing systems
+            HIPC(hipMemcpyAsync(nc, s->sys_cnt, sizeof(int) * SYSCNT * s->sys_cnt_cap,
+                                hipMemcpyDeviceToDevice, s->stream));
+            std::memcpy(nh, s->sys_cnt_host, sizeof(int) * SYSCNT * s->sys_cnt_cap);
+        }
+        HIPC(hipStreamSynchronize(s->stream));
+        dfree(s, s->sysd, s->sysd_cap);
+        dfree(s, s->sys_cnt, SYSCNT * s->sys_cnt_cap);
+        if (s->sys_cnt_host) (void)hipHostFree(s->sys_cnt_host);
+        s->sysd = nd;
+        s->sys_cnt = nc;
+        s->sys_cnt_host = nh;
+        s->sysd_cap = cap;
+        s->sys_cnt_cap = cap;
+    }
+    if (n > 0)
+        HIPC(hipMemcpyAsync(s->sysd, s->sys_host.data(), sizeof(SysDesc) * n,
+                            hipMemcpyHostToDevice, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+// Host mirrors of the device counters after a build.
 int sync_meta(sph_system* s)
 {
     HIPC(hipMemcpyAsync(s->meta_host, s->meta, sizeof(int) * META_SLOTS, hipMemcpyDeviceToHost,
                         s->stream));
+    if (nsys(s) > 0)
+        HIPC(hipMemcpyAsync(s->sys_cnt_host, s->sys_cnt, sizeof(int) * SYSCNT * nsys(s),
+                            hipMemcpyDeviceToHost, s->stream));
     HIPC(hipStreamSynchronize(s->stream));
     s->n = s->meta_host[META_N];
+    for (int i = 0; i < nsys(s); ++i) {
+        s->sys_n[i] = s->sys_cnt_host[SYSCNT * i + SC_N];
+        s->sys_off[i] = s->sys_cnt_host[SYSCNT * i + SC_OFF];
+    }
     return 0;
 }
 
-void prof_begin(sph_system* s, int k, hipEvent_t* ev)
+void prof_begin(sph_system* s, hipEvent_t* ev)
 {
     if (!s->prof) return;
     for (int i = 0; i < 2; ++i) {
         if (s->ev_pool.empty()) {
             hipEvent_t e;
-            hipEventCreate(&e);
+            (void)hipEventCreate(&e);
             s->ev_pool.push_back(e);
         }
         ev[i] = s->ev_pool.back();
         s->ev_pool.pop_back();
     }
-    hipEventRecord(ev[0], s->stream);
-    (void)k;
+    (void)hipEventRecord(ev[0], s->stream);
 }
 
 void prof_end(sph_system* s, int k, hipEvent_t* ev)
 {
     if (!s->prof) return;
-    hipEventRecord(ev[1], s->stream);
+    (void)hipEventRecord(ev[1], s->stream);
     s->ev_open.push_back({k, {ev[0], ev[1]}});
 }
 
 void prof_collect(sph_system* s)
 {
     if (s->ev_open.empty()) return;
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     for (auto& e : s->ev_open) {
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        (void)hipEventElapsedTime(&ms, e.second.first, e.second.second);
         s->prof_ms[e.first] += ms;
         s->prof_n[e.first] += 1;
         s->ev_pool.push_back(e.second.first);
@@ -198,11 +387,11 @@ int choose_tile(double occupancy)
     return T;
 }
 
-PairCtx make_ctx(sph_system* s, const Phys& ph)
+PairCtx make_ctx(sph_system* s)
 {
     PairCtx c{};
-    c.g = s->geo;
-    c.ph = ph;
+    c.sys = s->sysd;
+    c.cell_sys = s->cell_sys;
     c.T = s->tile_T;
     c.cap = (int)s->cap;
     c.tiles = s->tiles;
@@ -227,10 +416,10 @@ int ensure_masks(sph_system* s)
     HIPC(hipMemcpyAsync(&need, s->mask_total, sizeof(need), hipMemcpyDeviceToHost, s->stream));
     HIPC(hipStreamSynchronize(s->stream));
     if ((long long)need > s->mask_cap) {
-        dfree(s->masks);
+        dfree(s, s->masks, s->mask_cap);
         s->mask_cap = 0;
         const long long cap = (long long)need + (long long)need / 4 + 4096;
-        if (dalloc(&s->masks, cap)) {
+        if (dalloc(s, &s->masks, cap)) {
             s->masks = nullptr;  // no masks: the kernels keep testing
             g_err.clear();
             return 0;
@@ -245,24 +434,25 @@ int ensure_masks(sph_system* s)
 
 // Grid of the pair kernels: the tiles are taken from a work queue, so a
 // grid that fills the chip is enough.
-int pair_grid(const sph_system* s, long long ntiles)
+int pair_grid(long long ntiles)
 {
-    long long g = std::min<long long>(ntiles, 256 * 32);
+    const long long g = std::min<long long>(ntiles, 256 * 32);
     return (int)std::max<long long>(g, 1);
 }
 
-int run_pair(sph_system* s, int op, const Phys& ph, int self, int overwrite,
-             unsigned long long* counter, long long ntiles, int prof_kernel, int mask_mode)
+int run_pair(sph_system* s, int op, int self, int overwrite, unsigned long long* counter,
+             long long ntiles, int prof_kernel, int mask_mode, int only_flagged = 0)
 {
-    PairCtx c = make_ctx(s, ph);
+    PairCtx c = make_ctx(s);
     c.self = self;
     c.overwrite = overwrite;
     c.pair_counter = counter;
     c.mask_mode = s->use_masks ? mask_mode : MASK_TEST;
+    c.only_flagged = only_flagged;
     hipEvent_t ev[2];
-    prof_begin(s, prof_kernel, ev);
+    prof_begin(s, ev);
     reset_queue(s);
-    launch_pair(op, s->precision, c, pair_grid(s, ntiles), s->stream);
+    launch_pair(op, s->precision, s->rows, s->flat, c, pair_grid(ntiles), s->stream);
     prof_end(s, prof_kernel, ev);
     HIPC(hipGetLastError());
     return 0;
@@ -287,144 +477,146 @@ int col_of_field(int field, int* ncomp)
     return -1;
 }
 
-}  // namespace
+int destroy(sph_system* s);
 
-namespace sph {
-__global__ void k_reset_queue(int* meta) { meta[META_QUEUE] = 0; }
-void reset_queue(sph_system* s) { k_reset_queue<<<1, 1, 0, s->stream>>>(s->meta); }
-}  // namespace sph
-
-extern "C" {
-
-const char* sph_last_error(void) { return g_err.c_str(); }
-
-int sph_create(sph_system** out, double h, const double* bmin, const double* bmax, int precision)
+// A handle with its stream, counters and a first particle allocation, but
+// no system yet.
+int new_handle(int precision, sph_system** out)
 {
-    if (!out || !bmin || !bmax) return fail(SPH_EINVAL, "null argument");
     *out = nullptr;
-    if (!(h > 0.0) || !std::isfinite(h))
-        return fail(SPH_EINVAL, "invalid ParticleSystem declaration! (h must be a positive float)");
     if (precision != SPH_FP64 && precision != SPH_MIXED)
         return fail(SPH_EINVAL, "unknown precision mode");
-    Geo g{};
-    g.h = h;
-    g.inv_h = 1.0 / h;
-    g.s_thr = sqrt_threshold(h);
-    long long lim[3];
-    for (int d = 0; d < 3; ++d) {
-        if (!std::isfinite(bmin[d]) || !std::isfinite(bmax[d]) || bmin[d] > bmax[d])
-            return fail(SPH_EINVAL, "invalid bounding box");
-        g.bmin[d] = bmin[d];
-        g.bmax[d] = bmax[d];
-        // src/structs.jl:66-67
-        const double fmin = std::floor(bmin[d] / h), fmax = std::floor(bmax[d] / h);
-        if (std::fabs(fmin) > 1e15 || std::fabs(fmax) > 1e15)
-            return fail(SPH_EINVAL, "bounding box too large for h");
-        g.kp[d] = (long long)fmin;
-        lim[d] = (long long)fmax - g.kp[d] + 1;
-    }
-    const long long km = lim[0] * lim[1] * lim[2];
-    if (km <= 0 || km > (1ll << 30))
-        return fail(SPH_EINVAL, "cell grid too large (key_max > 2^30)");
-    g.L1 = (int)lim[0];
-    g.L2 = (int)lim[1];
-    g.L3 = (int)lim[2];
-    g.key_max = (int)km;
-    g.dim = lim[2] == 1 ? 2 : 3;  // src/structs.jl:70
-    g.flat = bmin[2] == bmax[2];
-
-    sph_system* s = new sph_system();
-    std::memset(s->prof_ms, 0, sizeof(s->prof_ms));
-    std::memset(s->prof_n, 0, sizeof(s->prof_n));
-    s->geo = g;
+    sph_system* s = new sph_system();  // value-initialised: pointers null, counts 0
     s->precision = precision;
     s->tile_T = 32;
     s->tile_auto = 1;
-    s->n = 0;
-    s->cap = 0;
-    s->cells_valid = false;
-    s->cur = 0;
-    s->cols[0] = s->cols[1] = nullptr;
-    s->icols[0] = s->icols[1] = nullptr;
-    s->slot = s->tmp_idx = s->tmp_key = s->tmp_ref = s->perm = s->inv = nullptr;
-    s->tiles = nullptr;
-    s->prho2 = s->stage = s->red_part = nullptr;
-    s->cell_chunks = nullptr;
-    s->mask_start = s->scan_part64 = s->masks = s->mask_total = nullptr;
-    s->mask_cap = 0;
-    s->masks_valid = false;
+    s->rows = 3;
+    s->flat = 1;
     s->use_masks = true;
-    s->prof = false;
-    s->step_pairs[0] = s->step_pairs[1] = 0;
-    int rc;
+    int rc = 0;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        delete s;
-        return fail(SPH_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        s->stream = nullptr;
+        rc = fail(SPH_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    s->scan_part_n = scan_partials_needed(km + 2);
-    if ((rc = dalloc(&s->meta, META_SLOTS)) || (rc = dalloc(&s->pair_counter, 4)) ||
-        (rc = dalloc(&s->cnt, km + 1)) || (rc = dalloc(&s->cell_start, km + 2)) ||
-        (rc = dalloc(&s->tile_start, km + 2)) || (rc = dalloc(&s->scan_part, s->scan_part_n)) ||
-        (rc = dalloc(&s->red_out, 4)) || (rc = dalloc(&s->cell_chunks, km + 1)) ||
-        (rc = dalloc(&s->mask_start, km + 2)) || (rc = dalloc(&s->scan_part64, s->scan_part_n)) ||
-        (rc = dalloc(&s->mask_total, 1))) {
-        sph_destroy(s);
+    if (!rc && !(rc = dalloc(s, &s->meta, META_SLOTS)) && !(rc = dalloc(s, &s->pair_counter, 4)) &&
+        !(rc = dalloc(s, &s->red_out, 4)) && !(rc = dalloc(s, &s->mask_total, 1))) {
+        e = hipHostMalloc((void**)&s->meta_host, sizeof(int) * META_SLOTS, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            s->meta_host = nullptr;
+            rc = fail(SPH_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+    }
+    if (!rc) {
+        std::memset(s->meta_host, 0, sizeof(int) * META_SLOTS);
+        e = hipMemsetAsync(s->meta, 0, sizeof(int) * META_SLOTS, s->stream);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(s->mask_total, 0, sizeof(unsigned long long), s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = fail(SPH_EHIP, std::string("init: ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = ensure_capacity(s, 1);
+    if (!rc) rc = upload_sysd(s);
+    if (rc) {
+        const std::string msg = g_err;
+        destroy(s);
+        g_err = msg;
         return rc;
-    }
-    e = hipHostMalloc((void**)&s->meta_host, sizeof(int) * META_SLOTS, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        s->meta_host = nullptr;
-        sph_destroy(s);
-        return fail(SPH_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-    }
-    std::memset(s->meta_host, 0, sizeof(int) * META_SLOTS);
-    e = hipMemsetAsync(s->meta, 0, sizeof(int) * META_SLOTS, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) {
-        sph_destroy(s);
-        return fail(SPH_EHIP, std::string("init: ") + hipGetErrorString(e));
     }
     *out = s;
     return 0;
 }
 
-int sph_destroy(sph_system* s)
+int destroy(sph_system* s)
 {
-    if (!s) return 0;
-    if (s->stream) hipStreamSynchronize(s->stream);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     free_particle_arrays(s);
-    dfree(s->meta); dfree(s->pair_counter); dfree(s->cnt); dfree(s->cell_start);
-    dfree(s->tile_start); dfree(s->scan_part); dfree(s->red_out);
-    dfree(s->cell_chunks); dfree(s->mask_start); dfree(s->scan_part64); dfree(s->masks);
-    dfree(s->mask_total);
-    if (s->meta_host) hipHostFree(s->meta_host);
-    for (auto e : s->ev_pool) hipEventDestroy(e);
+    free_cell_arrays(s);
+    dfree(s, s->meta, META_SLOTS); dfree(s, s->pair_counter, 4); dfree(s, s->red_out, 4);
+    dfree(s, s->masks, s->mask_cap); dfree(s, s->mask_total, 1);
+    dfree(s, s->sysd, s->sysd_cap); dfree(s, s->sys_cnt, SYSCNT * s->sys_cnt_cap);
+    if (s->meta_host) (void)hipHostFree(s->meta_host);
+    if (s->sys_cnt_host) (void)hipHostFree(s->sys_cnt_host);
+    for (auto e : s->ev_pool) (void)hipEventDestroy(e);
     for (auto& e : s->ev_open) {
-        hipEventDestroy(e.second.first);
-        hipEventDestroy(e.second.second);
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
     }
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return 0;
 }
 
-int sph_append(sph_system* s, int64_t n, const double* x, const double* v, const double* rho,
-               const double* P, const double* type)
+// Add a system at the end of the table; its particles follow.
+int add_system(sph_system* s, const Geo& g, const Phys& ph, int* slot_out)
 {
-    if (!s || n < 0 || (n > 0 && !x)) return fail(SPH_EINVAL, "invalid append arguments");
+    const int rows = g.dim == 2 ? 3 : 9;
+    if (nsys(s) > 0 && rows != s->rows)
+        return fail(SPH_EINVAL, "all systems of a batch must have the same dimension");
+    if (s->cells_total + g.key_max >= (1ll << 30))
+        return fail(SPH_EINVAL, "too many cells in the batch");
+    SysDesc d{};
+    d.g = g;
+    d.ph = ph;
+    d.cell_base = (int)s->cells_total;
+    d.live = 1;
+    d.pass_on = 0;
+    s->sys_host.push_back(d);
+    s->sys_n.push_back(0);
+    s->sys_off.push_back(s->n);
+    s->cells_total += g.key_max;
+    const int slot = nsys(s) - 1;
+    s->id_of_slot.push_back((int)s->slot_of_id.size());
+    s->slot_of_id.push_back(slot);
+    s->rho0_pending.push_back(1);
+    int rc = upload_sysd(s);
+    if (!rc) rc = ensure_cells(s, s->cells_total + nsys(s) + 1);
+    if (rc) {  // leave the table as it was
+        s->slot_of_id.back() = -1;
+        s->sys_host.pop_back();
+        s->sys_n.pop_back();
+        s->sys_off.pop_back();
+        s->id_of_slot.pop_back();
+        s->rho0_pending.pop_back();
+        s->cells_total -= g.key_max;
+        return rc;
+    }
+    s->rows = rows;
+    s->flat = slot == 0 ? g.flat : (s->flat && g.flat);
+    enqueue_fill_int(s, s->cell_sys + d.cell_base, slot, g.key_max);
+    enqueue_set_int(s, s->sys_cnt + SYSCNT * slot + SC_N, 0);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s->stream));
+    s->cells_valid = s->masks_valid = false;
+    *slot_out = slot;
+    return 0;
+}
+
+// Append n particles to system `slot`.  The device rows [off, off + n_sys)
+// of a system stay contiguous: appends go to the end of the arrays, which
+// is the end of the system when it is the last one holding particles.
+int append_particles(sph_system* s, int slot, int64_t n, const double* x, const double* v,
+                     const double* rho, const double* P, const double* type)
+{
+    if (n < 0 || (n > 0 && !x)) return fail(SPH_EINVAL, "invalid append arguments");
     if (n == 0) return 0;
+    if (s->sys_n[slot] > 0 && s->sys_off[slot] + s->sys_n[slot] != s->n)
+        return fail(SPH_EINVAL, "particles can only be appended to the last system of a batch");
+    if ((long long)n > (1ll << 31) - 64 - s->n) return fail(SPH_EINVAL, "particle count exceeds 2^31");
     int rc;
     if ((rc = ensure_capacity(s, s->n + n))) return rc;
-    // pack x, v, rho, P, type back to back into the staging buffer
+    // x, v, rho, P, type back to back in the staging buffer
     double* st = s->stage;
     HIPC(hipMemcpyAsync(st, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s->stream));
     if (v) HIPC(hipMemcpyAsync(st + 3 * n, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s->stream));
     if (rho) HIPC(hipMemcpyAsync(st + 6 * n, rho, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
     if (P) HIPC(hipMemcpyAsync(st + 7 * n, P, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
     if (type) HIPC(hipMemcpyAsync(st + 8 * n, type, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
-    enqueue_append(s, (int)s->n, (int)n, st, v != nullptr, rho != nullptr, P != nullptr,
-                   type != nullptr);
+    enqueue_append(s, (int)s->n, (int)n, (int)s->sys_n[slot], slot, st, v != nullptr,
+                   rho != nullptr, P != nullptr, type != nullptr);
+    if (s->sys_n[slot] == 0) s->sys_off[slot] = s->n;
+    s->sys_n[slot] += n;
+    enqueue_set_int(s, s->sys_cnt + SYSCNT * slot + SC_N, (int)s->sys_n[slot]);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
     s->n += n;
@@ -432,51 +624,16 @@ int sph_append(sph_system* s, int64_t n, const double* x, const double* v, const
     return 0;
 }
 
-int sph_count(sph_system* s, int64_t* n)
+int build_cells(sph_system* s)
 {
-    if (!s || !n) return fail(SPH_EINVAL, "null argument");
-    *n = s->n;
-    return 0;
-}
-
-int sph_set_field(sph_system* s, int field, const double* host, int64_t n)
-{
-    if (!s || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
-    int nc;
-    const int col = col_of_field(field, &nc);
-    if (col < 0) return fail(SPH_EINVAL, "unknown field");
-    if (n != s->n) return fail(SPH_EINVAL, "field length does not match the particle count");
-    if (n == 0) return 0;
-    HIPC(hipMemcpyAsync(s->stage, host, sizeof(double) * nc * n, hipMemcpyHostToDevice, s->stream));
-    enqueue_field_in(s, col, nc, s->stage);
-    HIPC(hipGetLastError());
-    HIPC(hipStreamSynchronize(s->stream));
-    if (field == SPH_FIELD_X) s->cells_valid = s->masks_valid = false;
-    return 0;
-}
-
-int sph_get_field(sph_system* s, int field, double* host, int64_t n)
-{
-    if (!s || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
-    int nc;
-    const int col = col_of_field(field, &nc);
-    if (col < 0) return fail(SPH_EINVAL, "unknown field");
-    if (n != s->n) return fail(SPH_EINVAL, "field length does not match the particle count");
-    if (n == 0) return 0;
-    enqueue_field_out(s, col, nc, s->stage);
-    HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(host, s->stage, sizeof(double) * nc * n, hipMemcpyDeviceToHost, s->stream));
-    HIPC(hipStreamSynchronize(s->stream));
-    return 0;
-}
-
-int sph_build_cells(sph_system* s, int64_t* n_out)
-{
-    if (!s) return fail(SPH_EINVAL, "null system");
     int rc;
-    if (s->cap == 0 && (rc = ensure_capacity(s, 1))) return rc;
+    if (nsys(s) == 0) {  // nothing to sort: no cell arrays yet
+        s->n = 0;
+        s->cells_valid = true;
+        return 0;
+    }
     hipEvent_t ev[2];
-    prof_begin(s, SPH_KERNEL_BUILD, ev);
+    prof_begin(s, ev);
     enqueue_build(s);
     prof_end(s, SPH_KERNEL_BUILD, ev);
     HIPC(hipGetLastError());
@@ -493,7 +650,376 @@ int sph_build_cells(sph_system* s, int64_t* n_out)
     if ((rc = ensure_masks(s))) return rc;
     s->cells_valid = true;
     s->masks_valid = false;
-    if (n_out) *n_out = s->n;
+    s->retired_since_build = false;
+    return 0;
+}
+
+// Single-system calls carry their parameters: they become slot 0's.  Every
+// caller synchronises the stream before returning, after which the host
+// table may change again.
+int set_params0(sph_system* s, const sph_params* p)
+{
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    s->sys_host[0].ph = make_phys(s->sys_host[0].g, p);
+    HIPC(hipMemcpyAsync(s->sysd, s->sys_host.data(), sizeof(SysDesc), hipMemcpyHostToDevice,
+                        s->stream));
+    return 0;
+}
+
+int run_steps(sph_system* s, int scheme, int nsteps)
+{
+    if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0)
+        return fail(SPH_EUNSUP, "unknown scheme");
+    if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
+    const int eos = scheme == SPH_SCHEME_WCSPH_SE ? SPH_OP_FIND_PRESSURE : SPH_OP_FIND_PRESSURE_RHO0;
+    int rc;
+    if (nsys(s) == 0) {
+        s->step_pairs[0] = s->step_pairs[1] = 0;
+        return 0;
+    }
+    // Tiles are taken from a work queue: the grid needs no tile count.
+    const long long tiles_bound = s->tile_cap;
+    HIPC(hipMemsetAsync(s->pair_counter, 0, 2 * sizeof(unsigned long long), s->stream));
+    for (int k = 0; k < nsteps; ++k) {
+        hipEvent_t ev[2];
+        prof_begin(s, ev);
+        enqueue_build(s);
+        prof_end(s, SPH_KERNEL_BUILD, ev);
+        if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
+                           MASK_TEST_STORE)))
+            return rc;
+        prof_begin(s, ev);
+        enqueue_unary(s, eos, s->prho2);
+        prof_end(s, SPH_KERNEL_UNARY, ev);
+        if ((rc = run_pair(s, P_FORCE, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE,
+                           MASK_LOAD)))
+            return rc;
+        prof_begin(s, ev);
+        enqueue_kick_drift(s);
+        prof_end(s, SPH_KERNEL_UNARY, ev);
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 2 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s->stream));
+    if ((rc = sync_meta(s))) return rc;
+    if (nsteps > 0) {
+        s->retired_since_build = false;
+        s->cells_valid = s->masks_valid = false;  // positions moved after the last build
+    }
+    return ensure_masks(s);  // size the masks for the next call
+}
+
+int slot_of(sph_system* s, int id, bool live_only = true)
+{
+    if (id < 0 || id >= (int)s->slot_of_id.size() || s->slot_of_id[id] < 0) return -1;
+    const int slot = s->slot_of_id[id];
+    if (live_only && !s->sys_host[slot].live) return -1;
+    return slot;
+}
+
+// Drop retired systems from the table once their particles have left the
+// cells (a build ran since they were retired): the live systems get new
+// slots and cell ranges, and the system column of the particles is
+// remapped.  The particle order is unchanged.
+int compact(sph_system* s)
+{
+    const int n0 = nsys(s);
+    bool any_dead = false;
+    for (int i = 0; i < n0; ++i) any_dead |= !s->sys_host[i].live;
+    if (!any_dead || s->retired_since_build) return 0;
+    int rc;
+    if ((rc = sync_meta(s))) return rc;
+    std::vector<int> map(n0, 0);
+    std::vector<SysDesc> nh;
+    std::vector<long long> nn, no;
+    std::vector<int> nid;
+    std::vector<char> npend;
+    long long cells = 0;
+    for (int i = 0; i < n0; ++i) {
+        if (!s->sys_host[i].live) continue;
+        map[i] = (int)nh.size();
+        SysDesc d = s->sys_host[i];
+        d.cell_base = (int)cells;
+        cells += d.g.key_max;
+        nh.push_back(d);
+        nn.push_back(s->sys_n[i]);
+        no.push_back(s->sys_off[i]);
+        nid.push_back(s->id_of_slot[i]);
+        npend.push_back(s->rho0_pending[i]);
+    }
+    int* dmap = nullptr;
+    if ((rc = dalloc(s, &dmap, n0))) return rc;
+    hipError_t e = hipMemcpyAsync(dmap, map.data(), sizeof(int) * n0, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        enqueue_remap_sys(s, dmap);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(s, dmap, n0);
+    if (e != hipSuccess) return fail(SPH_EHIP, std::string("compaction: ") + hipGetErrorString(e));
+    for (int i = 0; i < n0; ++i)
+        s->slot_of_id[s->id_of_slot[i]] = s->sys_host[i].live ? map[i] : -1;
+    s->sys_host = nh;
+    s->sys_n = nn;
+    s->sys_off = no;
+    s->id_of_slot = nid;
+    s->rho0_pending = npend;
+    s->cells_total = cells;
+    if (nsys(s) == 0) s->flat = 1;
+    if ((rc = upload_sysd(s))) return rc;
+    for (int i = 0; i < nsys(s); ++i) {
+        int* c = s->sys_cnt_host + SYSCNT * i;
+        c[SC_NTOT] = c[SC_N] = (int)s->sys_n[i];
+        c[SC_NREM] = 0;
+        c[SC_OFF] = (int)s->sys_off[i];
+        c[SC_REFBASE] = 0;
+    }
+    if (nsys(s) > 0)
+        HIPC(hipMemcpyAsync(s->sys_cnt, s->sys_cnt_host, sizeof(int) * SYSCNT * nsys(s),
+                            hipMemcpyHostToDevice, s->stream));
+    for (int i = 0; i < nsys(s); ++i)
+        enqueue_fill_int(s, s->cell_sys + s->sys_host[i].cell_base, i, s->sys_host[i].g.key_max);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(s->stream));
+    s->cells_valid = s->masks_valid = false;
+    return 0;
+}
+
+int field_io(sph_system* s, int slot, int field, double* host, int64_t n, bool in)
+{
+    int nc;
+    const int col = col_of_field(field, &nc);
+    if (col < 0) return fail(SPH_EINVAL, "unknown field");
+    if (n != s->sys_n[slot]) return fail(SPH_EINVAL, "field length does not match the particle count");
+    if (n == 0) return 0;
+    const long long off = s->sys_off[slot];
+    if (in) {
+        HIPC(hipMemcpyAsync(s->stage, host, sizeof(double) * nc * n, hipMemcpyHostToDevice, s->stream));
+        enqueue_field_in(s, off, n, col, nc, s->stage);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(s->stream));
+        if (field == SPH_FIELD_X) s->cells_valid = s->masks_valid = false;
+    } else {
+        enqueue_field_out(s, off, n, col, nc, s->stage);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(host, s->stage, sizeof(double) * nc * n, hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+    }
+    return 0;
+}
+
+int reduce_slot(sph_system* s, int slot, int what, double* out)
+{
+    if (what != SPH_REDUCE_ENERGY && what != SPH_REDUCE_ENERGY_COLLISION &&
+        what != SPH_REDUCE_MOMENTUM && what != SPH_REDUCE_ENERGY_RHO0)
+        return fail(SPH_EUNSUP, "unknown reduction");
+    double res[3] = {0.0, 0.0, 0.0};
+    if (s->sys_n[slot] > 0) {
+        enqueue_reduce(s, slot, s->sys_off[slot], s->sys_n[slot], what, s->red_out);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(res, s->red_out, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+    }
+    const int nv = what == SPH_REDUCE_MOMENTUM ? 3 : 1;
+    for (int i = 0; i < nv; ++i) out[i] = res[i];
+    return 0;
+}
+
+// Neighbour pairs of the current cell lists; with only_slot >= 0, of that
+// system only (the pair kernels skip the tiles of the others).
+int neighbour_pairs(sph_system* s, int only_slot, int64_t* pairs, int64_t cap, int64_t* n)
+{
+    *n = 0;
+    if (s->n == 0) return 0;
+    int rc;
+    const int flagged = only_slot >= 0;
+    if (flagged) {
+        for (int i = 0; i < nsys(s); ++i) s->sys_host[i].pass_on = i == only_slot;
+        if ((rc = upload_sysd(s))) return rc;
+    }
+    unsigned long long total = 0;
+    long long* dpairs = nullptr;
+    hipError_t e = hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream);
+    rc = e == hipSuccess ? 0 : fail(SPH_EHIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    if (!rc)
+        rc = run_pair(s, P_COUNT, 0, 0, s->pair_counter, s->meta_host[META_NTILES],
+                      SPH_KERNEL_DENSITY, MASK_TEST, flagged);
+    if (!rc) {
+        e = hipMemcpyAsync(&total, s->pair_counter, sizeof(total), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = fail(SPH_EHIP, std::string("pair count: ") + hipGetErrorString(e));
+    }
+    *n = (int64_t)total;
+    if (!rc && pairs && cap > 0 && total > 0) {
+        const long long ncopy = std::min<long long>(cap, (long long)total);
+        rc = dalloc(s, &dpairs, 2 * ncopy);
+        if (!rc) {
+            PairCtx c = make_ctx(s);
+            c.emit_counter = s->pair_counter;
+            c.pair_out = dpairs;
+            c.pair_cap = ncopy;
+            c.only_flagged = flagged;
+            e = hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream);
+            if (e == hipSuccess) {
+                reset_queue(s);
+                launch_pair(P_EMIT, s->precision, s->rows, s->flat, c,
+                            pair_grid(s->meta_host[META_NTILES]), s->stream);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(pairs, dpairs, sizeof(long long) * 2 * ncopy,
+                                   hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            dfree(s, dpairs, 2 * ncopy);
+            if (e != hipSuccess) rc = fail(SPH_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
+        }
+    }
+    if (flagged) {
+        for (int i = 0; i < nsys(s); ++i) s->sys_host[i].pass_on = 0;
+        const std::string msg = g_err;
+        const int rc2 = upload_sysd(s);
+        if (rc) g_err = msg;
+        else rc = rc2;
+    }
+    return rc;
+}
+
+// Times the density and force passes for every tile size on the current
+// cell lists with a zero mass (both passes then add exactly zero to rho
+// and a, so the state is left as it was) and keeps the fastest.
+int autotune(sph_system* s, int* best)
+{
+    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    std::vector<Phys> saved;
+    for (auto& d : s->sys_host) {
+        saved.push_back(d.ph);
+        d.ph.m = 0.0;
+        d.ph.kW = 0.0;
+        d.ph.kdW = 0.0;
+    }
+    int rc = upload_sysd(s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && hipEventCreate(&e0) != hipSuccess) rc = fail(SPH_EHIP, "hipEventCreate");
+    if (!rc && hipEventCreate(&e1) != hipSuccess) rc = fail(SPH_EHIP, "hipEventCreate");
+    float best_ms = 1e30f;
+    int best_T = s->tile_T;
+    const int cand[4] = {8, 16, 32, 64};
+    if (!rc) enqueue_prho2(s);
+    for (int i = 0; i < 4 && !rc; ++i) {
+        if ((rc = sph_set_tile(s, cand[i]))) break;
+        float tot = 0.0f;
+        for (int rep = 0; rep < 3 && !rc; ++rep) {
+            (void)hipEventRecord(e0, s->stream);
+            if ((rc = run_pair(s, P_DENSITY, 0, 0, nullptr, s->meta_host[META_NTILES],
+                               SPH_KERNEL_DENSITY, MASK_TEST_STORE)))
+                break;
+            if ((rc = run_pair(s, P_FORCE, 0, 0, nullptr, s->meta_host[META_NTILES],
+                               SPH_KERNEL_FORCE, MASK_LOAD)))
+                break;
+            (void)hipEventRecord(e1, s->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(SPH_EHIP, "hipEventSynchronize");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) tot += ms;
+        }
+        if (!rc && tot < best_ms) {
+            best_ms = tot;
+            best_T = cand[i];
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (size_t i = 0; i < s->sys_host.size(); ++i) s->sys_host[i].ph = saved[i];
+    const std::string msg = g_err;
+    const int rc2 = upload_sysd(s);
+    if (rc) {
+        g_err = msg;
+        return rc;
+    }
+    if (rc2) return rc2;
+    if ((rc = sph_set_tile(s, best_T))) return rc;
+    s->masks_valid = false;  // the timing passes ran with a zero mass
+    if (best) *best = best_T;
+    return 0;
+}
+
+}  // namespace
+
+namespace sph {
+__global__ void k_reset_queue(int* meta) { meta[META_QUEUE] = 0; }
+void reset_queue(sph_system* s) { k_reset_queue<<<1, 1, 0, s->stream>>>(s->meta); }
+}  // namespace sph
+
+extern "C" {
+
+const char* sph_last_error(void) { return g_err.c_str(); }
+
+// ================================================================ sph.h ==
+
+int sph_create(sph_system** out, double h, const double* bmin, const double* bmax, int precision)
+{
+    if (!out || !bmin || !bmax) return fail(SPH_EINVAL, "null argument");
+    *out = nullptr;
+    Geo g;
+    int rc = make_geo(h, bmin, bmax, 0, &g);
+    if (rc) return rc;
+    sph_system* s;
+    if ((rc = new_handle(precision, &s))) return rc;
+    int slot;
+    if ((rc = add_system(s, g, make_phys(g, nullptr), &slot))) {
+        const std::string msg = g_err;
+        destroy(s);
+        g_err = msg;
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int sph_destroy(sph_system* s)
+{
+    if (!s) return 0;
+    return destroy(s);
+}
+
+int sph_append(sph_system* s, int64_t n, const double* x, const double* v, const double* rho,
+               const double* P, const double* type)
+{
+    if (!s) return fail(SPH_EINVAL, "invalid append arguments");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return append_particles(s, 0, n, x, v, rho, P, type);
+}
+
+int sph_count(sph_system* s, int64_t* n)
+{
+    if (!s || !n) return fail(SPH_EINVAL, "null argument");
+    *n = nsys(s) > 0 ? s->sys_n[0] : 0;
+    return 0;
+}
+
+int sph_set_field(sph_system* s, int field, const double* host, int64_t n)
+{
+    if (!s || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return field_io(s, 0, field, const_cast<double*>(host), n, true);
+}
+
+int sph_get_field(sph_system* s, int field, double* host, int64_t n)
+{
+    if (!s || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return field_io(s, 0, field, host, n, false);
+}
+
+int sph_build_cells(sph_system* s, int64_t* n_out)
+{
+    if (!s) return fail(SPH_EINVAL, "null system");
+    int rc = build_cells(s);
+    if (rc) return rc;
+    if (n_out) *n_out = nsys(s) > 0 ? s->sys_n[0] : 0;
     return 0;
 }
 
@@ -504,7 +1030,8 @@ int sph_apply_binary(sph_system* s, int op, const sph_params* params, int self)
     if (!s->cells_valid)
         return fail(SPH_ESTATE, "cell list is not current: call sph_build_cells after moving or adding particles");
     if (s->n == 0) return 0;
-    const Phys ph = make_phys(s, params);
+    int rc = set_params0(s, params);
+    if (rc) return rc;
     int pop = P_DENSITY, kid = SPH_KERNEL_DENSITY;
     switch (op) {
     case SPH_OP_DENSITY_SUM: pop = P_DENSITY; break;
@@ -523,7 +1050,7 @@ int sph_apply_binary(sph_system* s, int op, const sph_params* params, int self)
     // A density pass records the pair masks of this cell list; the passes
     // after it read them.
     const int mode = dens ? MASK_TEST_STORE : (s->masks_valid ? MASK_LOAD : MASK_TEST);
-    int rc = run_pair(s, pop, ph, dens && self, 0, nullptr, s->meta_host[META_NTILES], kid, mode);
+    rc = run_pair(s, pop, dens && self, 0, nullptr, s->meta_host[META_NTILES], kid, mode);
     if (rc) return rc;
     HIPC(hipStreamSynchronize(s->stream));
     if (dens) s->masks_valid = s->use_masks && s->masks != nullptr;
@@ -534,10 +1061,11 @@ int sph_apply_unary(sph_system* s, int op, const sph_params* params)
 {
     if (!s || !params) return fail(SPH_EINVAL, "null argument");
     if (!is_unary(op)) return fail(SPH_EUNSUP, "not a registered unary operator");
-    const Phys ph = make_phys(s, params);
+    int rc = set_params0(s, params);
+    if (rc) return rc;
     hipEvent_t ev[2];
-    prof_begin(s, SPH_KERNEL_UNARY, ev);
-    enqueue_unary(s, op, ph, nullptr);
+    prof_begin(s, ev);
+    enqueue_unary(s, op, nullptr);
     prof_end(s, SPH_KERNEL_UNARY, ev);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
@@ -555,98 +1083,26 @@ int sph_apply(sph_system* s, int op, const sph_params* params, int self)
 int sph_reduce(sph_system* s, int what, const sph_params* params, double* out)
 {
     if (!s || !params || !out) return fail(SPH_EINVAL, "null argument");
-    if (what != SPH_REDUCE_ENERGY && what != SPH_REDUCE_ENERGY_COLLISION &&
-        what != SPH_REDUCE_MOMENTUM && what != SPH_REDUCE_ENERGY_RHO0)
-        return fail(SPH_EUNSUP, "unknown reduction");
-    const Phys ph = make_phys(s, params);
-    double res[3] = {0.0, 0.0, 0.0};
-    if (s->n > 0) {
-        enqueue_reduce(s, what, ph, s->red_out);
-        HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(res, s->red_out, sizeof(res), hipMemcpyDeviceToHost, s->stream));
-        HIPC(hipStreamSynchronize(s->stream));
-    }
-    const int nv = what == SPH_REDUCE_MOMENTUM ? 3 : 1;
-    for (int i = 0; i < nv; ++i) out[i] = res[i];
-    return 0;
+    int rc = set_params0(s, params);
+    if (rc) return rc;
+    return reduce_slot(s, 0, what, out);
 }
 
 int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
 {
     if (!s || !n) return fail(SPH_EINVAL, "null argument");
     if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
-    *n = 0;
-    if (s->n == 0) return 0;
-    sph_params zero{};
-    const Phys ph = make_phys(s, &zero);
-    unsigned long long total = 0;
-    HIPC(hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream));
-    int rc = run_pair(s, P_COUNT, ph, 0, 0, s->pair_counter, s->meta_host[META_NTILES],
-                      SPH_KERNEL_DENSITY, MASK_TEST);
-    if (rc) return rc;
-    HIPC(hipMemcpyAsync(&total, s->pair_counter, sizeof(total), hipMemcpyDeviceToHost, s->stream));
-    HIPC(hipStreamSynchronize(s->stream));
-    *n = (int64_t)total;
-    if (!pairs || cap <= 0 || total == 0) return 0;
-    const long long ncopy = std::min<long long>(cap, (long long)total);
-    long long* dpairs = nullptr;
-    if ((rc = dalloc(&dpairs, 2 * ncopy))) return rc;
-    HIPC(hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream));
-    PairCtx c = make_ctx(s, ph);
-    c.emit_counter = s->pair_counter;
-    c.pair_out = dpairs;
-    c.pair_cap = ncopy;
-    reset_queue(s);
-    launch_pair(P_EMIT, s->precision, c, pair_grid(s, s->meta_host[META_NTILES]), s->stream);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(pairs, dpairs, sizeof(long long) * 2 * ncopy, hipMemcpyDeviceToHost,
-                           s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    hipFree(dpairs);
-    if (e != hipSuccess) return fail(SPH_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
-    return 0;
+    return neighbour_pairs(s, -1, pairs, cap, n);
 }
 
 int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, int64_t* pairs)
 {
     if (!s || !params) return fail(SPH_EINVAL, "null argument");
-    if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0)
-        return fail(SPH_EUNSUP, "unknown scheme");
-    const int eos = scheme == SPH_SCHEME_WCSPH_SE ? SPH_OP_FIND_PRESSURE : SPH_OP_FIND_PRESSURE_RHO0;
-    if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
-    int rc;
-    if (s->cap == 0 && (rc = ensure_capacity(s, 1))) return rc;
     sph_params p1 = *params;
     p1.coeff = 1.0;
-    const Phys ph = make_phys(s, &p1);
-    // Tiles are taken from a work queue: the grid needs no tile count.
-    const long long tiles_bound = s->tile_cap;
-    HIPC(hipMemsetAsync(s->pair_counter, 0, 2 * sizeof(unsigned long long), s->stream));
-    for (int k = 0; k < nsteps; ++k) {
-        hipEvent_t ev[2];
-        prof_begin(s, SPH_KERNEL_BUILD, ev);
-        enqueue_build(s);
-        prof_end(s, SPH_KERNEL_BUILD, ev);
-        if ((rc = run_pair(s, P_DENSITY, ph, 1, 1, s->pair_counter, tiles_bound,
-                           SPH_KERNEL_DENSITY, MASK_TEST_STORE)))
-            return rc;
-        prof_begin(s, SPH_KERNEL_UNARY, ev);
-        enqueue_unary(s, eos, ph, s->prho2);
-        prof_end(s, SPH_KERNEL_UNARY, ev);
-        if ((rc = run_pair(s, P_FORCE, ph, 0, 1, s->pair_counter + 1, tiles_bound,
-                           SPH_KERNEL_FORCE, MASK_LOAD)))
-            return rc;
-        prof_begin(s, SPH_KERNEL_UNARY, ev);
-        enqueue_kick_drift(s, ph);
-        prof_end(s, SPH_KERNEL_UNARY, ev);
-        HIPC(hipGetLastError());
-    }
-    HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 2 * sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, s->stream));
-    if ((rc = sync_meta(s))) return rc;
-    s->cells_valid = s->masks_valid = false;  // positions moved after the last build
-    if ((rc = ensure_masks(s))) return rc;    // size the masks for the next call
+    int rc = set_params0(s, &p1);
+    if (rc) return rc;
+    if ((rc = run_steps(s, scheme, nsteps))) return rc;
     if (pairs) *pairs = (int64_t)s->step_pairs[0];
     return 0;
 }
@@ -663,7 +1119,8 @@ int sph_key_geometry(sph_system* s, int64_t* key_phase, int64_t* key_lim, int64_
                      int64_t* key_diff, int* n_diff)
 {
     if (!s) return fail(SPH_EINVAL, "null system");
-    const Geo& g = s->geo;
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    const Geo& g = s->sys_host[0].g;
     const long long lim[3] = {g.L1, g.L2, g.L3};
     for (int d = 0; d < 3; ++d) {
         if (key_phase) key_phase[d] = g.kp[d];
@@ -713,50 +1170,9 @@ int sph_set_tile(sph_system* s, int T)
 int sph_autotune(sph_system* s, const sph_params* params, int* best)
 {
     if (!s || !params) return fail(SPH_EINVAL, "null argument");
-    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
-    // Time the density and force passes for every tile size.  A zero mass
-    // makes both passes add exactly zero to rho and a, so the state is left
-    // as it was while the full pair arithmetic still runs.
-    sph_params p0 = *params;
-    p0.m = 0.0;
-    const Phys ph = make_phys(s, &p0);
-    hipEvent_t e0, e1;
-    HIPC(hipEventCreate(&e0));
-    HIPC(hipEventCreate(&e1));
-    float best_ms = 1e30f;
-    int best_T = s->tile_T;
-    const int cand[4] = {8, 16, 32, 64};
-    enqueue_prho2(s);
-    for (int i = 0; i < 4; ++i) {
-        int rc = sph_set_tile(s, cand[i]);
-        if (rc) return rc;
-        float tot = 0.0f;
-        for (int rep = 0; rep < 3; ++rep) {
-            HIPC(hipEventRecord(e0, s->stream));
-            if ((rc = run_pair(s, P_DENSITY, ph, 0, 0, nullptr, s->meta_host[META_NTILES],
-                               SPH_KERNEL_DENSITY, MASK_TEST_STORE)))
-                return rc;
-            if ((rc = run_pair(s, P_FORCE, ph, 0, 0, nullptr, s->meta_host[META_NTILES],
-                               SPH_KERNEL_FORCE, MASK_LOAD)))
-                return rc;
-            HIPC(hipEventRecord(e1, s->stream));
-            HIPC(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            HIPC(hipEventElapsedTime(&ms, e0, e1));
-            if (rep > 0) tot += ms;
-        }
-        if (tot < best_ms) {
-            best_ms = tot;
-            best_T = cand[i];
-        }
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    int rc = sph_set_tile(s, best_T);
+    int rc = set_params0(s, params);
     if (rc) return rc;
-    s->masks_valid = false;  // the timing passes ran with a zero mass
-    if (best) *best = best_T;
-    return 0;
+    return autotune(s, best);
 }
 
 int sph_set_option(sph_system* s, int option, int value)
@@ -778,10 +1194,10 @@ int sph_candidates(sph_system* s, int64_t* all, int64_t* fluid)
     if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
     unsigned long long res[2] = {0, 0};
     if (s->n > 0) {
-        sph_params zero{};
-        PairCtx c = make_ctx(s, make_phys(s, &zero));
+        PairCtx c = make_ctx(s);
         HIPC(hipMemsetAsync(s->pair_counter, 0, 2 * sizeof(unsigned long long), s->stream));
-        launch_candidates(c, s->icols[s->cur] + I_KEY * s->cap, s->pair_counter, s->n, s->stream);
+        launch_candidates(s->rows, c, s->icols[s->cur] + I_KEY * s->cap, s->pair_counter, s->n,
+                          s->stream);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(res, s->pair_counter, sizeof(res), hipMemcpyDeviceToHost, s->stream));
         HIPC(hipStreamSynchronize(s->stream));
@@ -807,6 +1223,181 @@ int sph_profile_read(sph_system* s, int kernel, double* total_ms, int64_t* launc
     prof_collect(s);
     if (total_ms) *total_ms = s->prof_ms[kernel];
     if (launches) *launches = s->prof_n[kernel];
+    return 0;
+}
+
+// ========================================================== sph_batch.h ==
+
+int sph_batch_create(sph_batch** out, int precision)
+{
+    if (!out) return fail(SPH_EINVAL, "null argument");
+    return new_handle(precision, out);
+}
+
+int sph_batch_destroy(sph_batch* b) { return sph_destroy(b); }
+
+int sph_batch_add(sph_batch* b, const sph_system_desc* desc, int64_t n, const double* x,
+                  const double* v, const double* rho, const double* P, const double* type,
+                  int* id)
+{
+    if (!b || !desc || !id) return fail(SPH_EINVAL, "null argument");
+    if (n < 0 || (n > 0 && !x)) return fail(SPH_EINVAL, "invalid particle arguments");
+    Geo g;
+    int rc = make_geo(desc->h, desc->bbox_min, desc->bbox_max, desc->periodic, &g);
+    if (rc) return rc;
+    if ((rc = compact(b))) return rc;
+    sph_params p = desc->params;
+    p.coeff = 1.0;  // the fused step advances by dt
+    int slot;
+    if ((rc = add_system(b, g, make_phys(g, &p), &slot))) return rc;
+    if ((rc = append_particles(b, slot, n, x, v, rho, P, type))) {
+        // the system never held particles: retire it
+        const std::string msg = g_err;
+        b->sys_host[slot].live = 0;
+        b->slot_of_id[b->id_of_slot[slot]] = -1;
+        (void)upload_sysd(b);
+        g_err = msg;
+        return rc;
+    }
+    *id = b->id_of_slot[slot];
+    return 0;
+}
+
+int sph_batch_retire(sph_batch* b, int id)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    // The device counters keep the system's particle count until the next
+    // build moves its particles out of the cells (the removal replay of
+    // the other systems relies on the per-system ranges).
+    b->sys_host[slot].live = 0;
+    b->retired_since_build = true;
+    b->cells_valid = b->masks_valid = false;
+    return upload_sysd(b);
+}
+
+int sph_batch_size(sph_batch* b, int* nsys_live, int64_t* particles)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    int live = 0;
+    long long np = 0;
+    for (int i = 0; i < nsys(b); ++i)
+        if (b->sys_host[i].live) {
+            ++live;
+            np += b->sys_n[i];
+        }
+    if (nsys_live) *nsys_live = live;
+    if (particles) *particles = np;
+    return 0;
+}
+
+int sph_batch_count(sph_batch* b, int id, int64_t* n)
+{
+    if (!b || !n) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    *n = b->sys_n[slot];
+    return 0;
+}
+
+int sph_batch_get_field(sph_batch* b, int id, int field, double* host, int64_t n)
+{
+    if (!b || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return field_io(b, slot, field, host, n, false);
+}
+
+int sph_batch_set_field(sph_batch* b, int id, int field, const double* host, int64_t n)
+{
+    if (!b || (!host && n > 0)) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return field_io(b, slot, field, const_cast<double*>(host), n, true);
+}
+
+int sph_batch_reduce(sph_batch* b, int id, int what, double* out)
+{
+    if (!b || !out) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return reduce_slot(b, slot, what, out);
+}
+
+int sph_batch_build_cells(sph_batch* b)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    int rc = compact(b);
+    if (rc) return rc;
+    return build_cells(b);
+}
+
+int sph_batch_neighbour_pairs(sph_batch* b, int id, int64_t* pairs, int64_t cap, int64_t* n)
+{
+    if (!b || !n) return fail(SPH_EINVAL, "null argument");
+    if (!b->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return neighbour_pairs(b, slot, pairs, cap, n);
+}
+
+int sph_batch_init_rho0(sph_batch* b)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    int rc = compact(b);
+    if (rc) return rc;
+    bool any = false;
+    for (int i = 0; i < nsys(b); ++i) {
+        b->sys_host[i].pass_on = b->sys_host[i].live && b->rho0_pending[i];
+        any |= b->sys_host[i].pass_on != 0;
+    }
+    if (!any) return 0;
+    if ((rc = upload_sysd(b)) == 0 && (rc = build_cells(b)) == 0) {
+        // rho0_p = sum_q m W + m W(h, 0), written (not accumulated), for the
+        // flagged systems only
+        rc = run_pair(b, P_DENSITY_RHO0, 1, 1, nullptr, b->meta_host[META_NTILES],
+                      SPH_KERNEL_DENSITY, MASK_TEST, 1);
+        if (!rc && hipStreamSynchronize(b->stream) != hipSuccess)
+            rc = fail(SPH_EHIP, "reference density pass failed");
+    }
+    for (int i = 0; i < nsys(b); ++i) {
+        if (!rc && b->sys_host[i].pass_on) b->rho0_pending[i] = 0;
+        b->sys_host[i].pass_on = 0;
+    }
+    const std::string msg = g_err;
+    const int rc2 = upload_sysd(b);
+    if (rc) {
+        g_err = msg;
+        return rc;
+    }
+    return rc2;
+}
+
+int sph_batch_step(sph_batch* b, int scheme, int nsteps, int64_t* pairs)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    int rc = compact(b);
+    if (rc) return rc;
+    if ((rc = run_steps(b, scheme, nsteps))) return rc;
+    if (pairs) *pairs = (int64_t)b->step_pairs[0];
+    return 0;
+}
+
+int sph_batch_set_tile(sph_batch* b, int targets_per_tile) { return sph_set_tile(b, targets_per_tile); }
+
+int sph_batch_autotune(sph_batch* b, int* best)
+{
+    if (!b) return fail(SPH_EINVAL, "null batch");
+    int rc = b->cells_valid ? 0 : sph_batch_build_cells(b);
+    if (rc) return rc;
+    return autotune(b, best);
+}
+
+int sph_batch_footprint(sph_batch* b, int64_t* bytes)
+{
+    if (!b || !bytes) return fail(SPH_EINVAL, "null argument");
+    *bytes = b->bytes;
     return 0;
 }
 

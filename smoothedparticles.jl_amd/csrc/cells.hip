@@ -1,27 +1,30 @@
-// Cell-list builder: the device form of create_cell_list! (src/core.jl:51-90).
+// Cell-list builder: the device form of create_cell_list! (src/core.jl:51-90)
+// for every system of a batch at once.
 //
-// Stages, all on the system's stream, no host round trip:
-//   1. k_keys     FP64 find_key (src/structs.jl:97-106) and the inclusive
-//                 bounding-box test (src/geometry.jl:24-30); particles
-//                 outside get the sentinel key key_max.  Per-cell counts by
-//                 wave-aggregated atomics (one atomic per distinct key in a
-//                 wave; the input is the previous sorted order, so a wave
-//                 usually touches one or two cells).
+// Stages, all on the handle's stream, no host round trip:
+//   1. k_keys     per particle: FP64 find_key (src/structs.jl:97-106) and the
+//                 inclusive bounding-box test (src/geometry.jl:24-30); a
+//                 particle outside goes to its system's removal bucket.
+//                 Periodic systems wrap positions into the box instead.
+//                 Global key = the system's cell_base + local key.  Per-bucket
+//                 counts by wave-aggregated atomics (one atomic per distinct
+//                 key in a wave; the input is the previous sorted order, so a
+//                 wave usually touches one or two cells).
 //   2. scan       exclusive prefix sum of the counts -> cell_start
-//   3. k_scatter  bucket every particle into its cell range
-//   4. k_rank     order each cell by descending reference index, the order
+//   3. k_scatter  bucket every particle
+//   4. k_rank     order each bucket by descending reference index, the order
 //                 add_index! keeps (src/core.jl:26-41); deterministic
 //                 whatever order the atomics handed out
-//   5. k_gather   permute every column into the new cell-sorted order
-//   6. removal    particles outside the box sit in the sentinel bucket at
-//                 the end, already sorted by descending reference index; the
-//                 reference's swap-with-end renumbering (src/core.jl:72-81)
-//                 is replayed on their reference indices
-//   7. tiles      a scan of ceil(count/T) per cell and the tile table of the
-//                 pair kernels: each tile is up to T targets of one cell
+//   5. k_gather   permute every column into the new order
+//   6. removal    each system's removal bucket holds its removed particles in
+//                 descending reference index; the reference's swap-with-end
+//                 renumbering (src/core.jl:72-81) is replayed per system
+//   7. tiles      a scan of ceil(count/T) per cell, the 64-candidate chunks a
+//                 target of each cell walks, a scan of pair-mask words, and
+//                 the tile table of the pair kernels
 //
-// This is a counting sort on the cell key (one radix digit of
-// ceil(log2(key_max+1)) bits) followed by an in-bucket rank.
+// This is a counting sort on the global cell key (one radix digit) followed
+// by an in-bucket rank.
 #include "sph_internal.h"
 
 namespace sph {
@@ -35,33 +38,76 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, long long n)
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------- prepare --
+// Per-system reference-index base of the inverse map and the count at the
+// start of the build.
+__global__ void k_sys_prep(int nsys, int* sc, int* meta)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int base = 0;
+    for (int s = 0; s < nsys; ++s) {
+        sc[SYSCNT * s + SC_NTOT] = sc[SYSCNT * s + SC_N];
+        sc[SYSCNT * s + SC_REFBASE] = base;
+        base += sc[SYSCNT * s + SC_N];
+    }
+    meta[META_NTOT] = meta[META_N];
+}
+
 // ------------------------------------------------------------------ keys --
-__global__ __launch_bounds__(256) void k_keys(Geo g, const double* __restrict__ x,
-                                              const double* __restrict__ y,
-                                              const double* __restrict__ z, int* meta,
+__global__ __launch_bounds__(256) void k_keys(const SysDesc* __restrict__ sysd, int nsys,
+                                              int cells_total, double* x, double* y, double* z,
+                                              const int* __restrict__ sysid, const int* meta,
                                               int* __restrict__ key, int* __restrict__ slot,
                                               unsigned* __restrict__ cnt)
 {
     const int n = meta[META_N];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) meta[META_NTOT] = n;
     int k = -1;
     if (i < n) {
-        const double px = x[i], py = y[i], pz = z[i];
-        const bool inside = g.bmin[0] <= px && px <= g.bmax[0] && g.bmin[1] <= py &&
-                            py <= g.bmax[1] && g.bmin[2] <= pz && pz <= g.bmax[2];
-        if (inside) {
-            long long ci = (long long)floor(px / g.h) - g.kp[0];
-            long long cj = (long long)floor(py / g.h) - g.kp[1];
-            long long ck = (long long)floor(pz / g.h) - g.kp[2];
-            // In range by monotonicity of the correctly rounded division;
-            // the clamp only guards memory.
-            ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
-            cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
-            ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
-            k = (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+        const int s = sysid[i];
+        const SysDesc& d = sysd[s];
+        const Geo& g = d.g;
+        double px = x[i], py = y[i], pz = z[i];
+        if (!d.live) {
+            k = cells_total + nsys;
+        } else if (g.periodic) {
+            // positions live in [bmin, bmin + box) on the periodic axes
+            if (!(isfinite(px) && isfinite(py) && isfinite(pz))) {
+                k = cells_total + s;
+            } else {
+                px = px - g.box[0] * floor((px - g.bmin[0]) / g.box[0]);
+                py = py - g.box[1] * floor((py - g.bmin[1]) / g.box[1]);
+                x[i] = px;
+                y[i] = py;
+                long long ci = (long long)floor((px - g.bmin[0]) / g.cell[0]);
+                long long cj = (long long)floor((py - g.bmin[1]) / g.cell[1]);
+                long long ck = 0;
+                if (g.L3 > 1) {
+                    pz = pz - g.box[2] * floor((pz - g.bmin[2]) / g.box[2]);
+                    z[i] = pz;
+                    ck = (long long)floor((pz - g.bmin[2]) / g.cell[2]);
+                }
+                ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
+                cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
+                ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
+                k = d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+            }
         } else {
-            k = g.key_max;
+            const bool inside = g.bmin[0] <= px && px <= g.bmax[0] && g.bmin[1] <= py &&
+                                py <= g.bmax[1] && g.bmin[2] <= pz && pz <= g.bmax[2];
+            if (inside) {
+                long long ci = (long long)floor(px / g.h) - g.kp[0];
+                long long cj = (long long)floor(py / g.h) - g.kp[1];
+                long long ck = (long long)floor(pz / g.h) - g.kp[2];
+                // In range by monotonicity of the correctly rounded division;
+                // the clamp only guards memory.
+                ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
+                cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
+                ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
+                k = d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+            } else {
+                k = cells_total + s;
+            }
         }
         key[i] = k;
     }
@@ -124,11 +170,11 @@ struct SrcCount {
 struct SrcTiles {
     using V = unsigned;
     const int* cs;
-    int km;
+    int ncells;
     int T;
     __device__ V operator()(int i) const
     {
-        if (i >= km) return 0u;
+        if (i >= ncells) return 0u;
         const int n = cs[i + 1] - cs[i];
         return (unsigned)((n + T - 1) / T);
     }
@@ -138,11 +184,11 @@ struct SrcMaskWords {
     using V = unsigned long long;
     const int* tile_start;
     const int* chunks;
-    int km;
+    int ncells;
     int T;
     __device__ V operator()(int i) const
     {
-        if (i >= km) return 0ull;
+        if (i >= ncells) return 0ull;
         const long long nt = tile_start[i + 1] - tile_start[i];
         return (V)(nt * chunks[i] * T);
     }
@@ -256,54 +302,77 @@ __global__ __launch_bounds__(256) void k_gather(const int* meta, const int* __re
     for (int f = 0; f < NICOL; ++f) idst[f * cap + k] = isrc[f * cap + j];
 }
 
-__global__ void k_finalize(int* meta, const int* cell_start, int key_max)
+// Totals of the build, per system and overall.
+__global__ void k_finalize(const SysDesc* __restrict__ sysd, int nsys, int cells_total, int* sc,
+                           int* meta, const int* cell_start)
 {
-    const int ntot = meta[META_NTOT];
-    const int na = cell_start[key_max];
-    meta[META_NREM] = ntot - na;
-    meta[META_N] = na;
-    meta[META_NCELLS] = 0;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        const int na = cell_start[cells_total];
+        meta[META_NREM] = meta[META_NTOT] - na;
+        meta[META_N] = na;
+        meta[META_NCELLS] = 0;
+    }
+    if (s >= nsys) return;
+    const SysDesc& d = sysd[s];
+    const int off = cell_start[d.cell_base];
+    const int end = cell_start[d.cell_base + d.g.key_max];
+    sc[SYSCNT * s + SC_OFF] = off;
+    sc[SYSCNT * s + SC_N] = d.live ? end - off : 0;
+    sc[SYSCNT * s + SC_NREM] = cell_start[cells_total + s + 1] - cell_start[cells_total + s];
 }
 
 // Removal renumbering (src/core.jl:72-81): with the removed reference
-// indices r_1 > r_2 > ... > r_m, particles[r_i] = particles[N+1-i] in turn.
-// inv maps a reference position to the sorted slot that holds it.
+// indices r_1 > r_2 > ... > r_m of a system, particles[r_i] =
+// particles[N+1-i] in turn.  inv maps (system reference base + reference
+// position) to the sorted slot that holds it.
 __global__ __launch_bounds__(256) void k_inv(const int* meta, const int* __restrict__ ref,
+                                             const int* __restrict__ sysid, const int* sc,
                                              int* inv)
 {
     if (meta[META_NREM] == 0) return;
     const int n = meta[META_NTOT];
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < n) inv[ref[k]] = k;
+    if (k < n) inv[sc[SYSCNT * sysid[k] + SC_REFBASE] + ref[k]] = k;
 }
 
-__global__ void k_remove_seq(const int* meta, int* ref, int* inv)
+__global__ void k_remove_seq(const int* meta, int nsys, int cells_total, const int* sc,
+                             const int* cell_start, int* ref, int* inv)
 {
-    const int m = meta[META_NREM];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys || meta[META_NREM] == 0) return;
+    const int m = sc[SYSCNT * s + SC_NREM];
     if (m == 0) return;
-    const int na = meta[META_N], ntot = meta[META_NTOT];
+    const int first = cell_start[cells_total + s];
+    const int ntot = sc[SYSCNT * s + SC_NTOT];
+    const int rb = sc[SYSCNT * s + SC_REFBASE];
     for (int i = 0; i < m; ++i) {
-        const int r = ref[na + i];     // descending
+        const int r = ref[first + i];  // descending
         const int src = ntot - 1 - i;  // reference position N+1-i (1-based)
-        const int slot = inv[src];
-        inv[r] = slot;
+        const int slot = inv[rb + src];
+        inv[rb + r] = slot;
         ref[slot] = r;
     }
 }
 
-// 64-candidate chunks one target of cell c walks (same loop as the pair
-// kernels: per stencil row, ceil(row length / 64)).
+// 64-candidate chunks one target of global cell c walks (the same loop as
+// the pair kernels: per stencil row and range, ceil(length / 64)).
 template <int ROWS>
-__global__ __launch_bounds__(256) void k_cell_chunks(Geo g, const int* __restrict__ cell_start,
+__global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__ sysd,
+                                                     const int* __restrict__ cell_sys,
+                                                     int ncells, const int* __restrict__ cell_start,
                                                      int* chunks)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= g.key_max) return;
+    if (c >= ncells) return;
     int n = 0;
     if (cell_start[c + 1] > cell_start[c]) {
+        const SysDesc& d = sysd[cell_sys[c]];
+        const int lc = c - d.cell_base;
         for (int row = 0; row < ROWS; ++row) {
-            int j0, j1;
-            if (row_range<ROWS>(g, cell_start, c, row, j0, j1)) n += (j1 - j0 + 63) / 64;
+            Seg seg[2];
+            const int ns = row_segments<ROWS>(d, cell_start, lc, row, seg);
+            for (int q = 0; q < ns; ++q) n += (seg[q].j1 - seg[q].j0 + 63) / 64;
         }
     }
     chunks[c] = n;
@@ -315,17 +384,17 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
                                                const int* __restrict__ tile_start,
                                                const int* __restrict__ chunks,
                                                const unsigned long long* __restrict__ mask_start,
-                                               int key_max, int T, long long mask_cap, int4* tiles,
+                                               int ncells, int T, long long mask_cap, int4* tiles,
                                                int* meta, unsigned long long* mask_total)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c == 0) {
-        meta[META_NTILES] = tile_start[key_max];
-        const unsigned long long need = mask_start[key_max];
+        meta[META_NTILES] = tile_start[ncells];
+        const unsigned long long need = mask_start[ncells];
         *mask_total = need;
         meta[META_MASK_OK] = (long long)need <= mask_cap ? 1 : 0;
     }
-    if (c >= key_max) return;
+    if (c >= ncells) return;
     const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
     if (nt > 0) atomicAdd(&meta[META_NCELLS], 1);
     const int cs = cell_start[c];
@@ -342,32 +411,39 @@ static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 
 
 void enqueue_tiles(sph_system* s)
 {
-    const Geo& g = s->geo;
-    const unsigned nbc = nblocks(g.key_max + 1, 256);
-    scan(s, SrcTiles{s->cell_start, g.key_max, s->tile_T}, g.key_max + 1, s->tile_start,
-         s->scan_part);
-    if (g.dim == 2) k_cell_chunks<3><<<nbc, 256, 0, s->stream>>>(g, s->cell_start, s->cell_chunks);
-    else k_cell_chunks<9><<<nbc, 256, 0, s->stream>>>(g, s->cell_start, s->cell_chunks);
-    scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, g.key_max, s->tile_T}, g.key_max + 1,
+    const int ncells = (int)s->cells_total;
+    const unsigned nbc = nblocks(ncells + 1, 256);
+    scan(s, SrcTiles{s->cell_start, ncells, s->tile_T}, ncells + 1, s->tile_start, s->scan_part);
+    if (s->rows == 3)
+        k_cell_chunks<3><<<nbc, 256, 0, s->stream>>>(s->sysd, s->cell_sys, ncells, s->cell_start,
+                                                     s->cell_chunks);
+    else
+        k_cell_chunks<9><<<nbc, 256, 0, s->stream>>>(s->sysd, s->cell_sys, ncells, s->cell_start,
+                                                     s->cell_chunks);
+    scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, ncells, s->tile_T}, ncells + 1,
          s->mask_start, s->scan_part64);
     k_tiles<<<nbc, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
-                                        s->mask_start, g.key_max, s->tile_T,
+                                        s->mask_start, ncells, s->tile_T,
                                         s->masks ? s->mask_cap : 0, s->tiles, s->meta,
                                         s->mask_total);
 }
 
 void enqueue_build(sph_system* s)
 {
-    const Geo& g = s->geo;
     hipStream_t st = s->stream;
     const long long cap = s->cap;
+    const int nsys = (int)s->sys_host.size();
+    const int ncells = (int)s->cells_total;
+    const long long nbuckets = s->cells_total + nsys + 1;
     const unsigned nb = nblocks(s->n > 0 ? s->n : 1, 256);
     double* c = s->cols[s->cur];
     int* ic = s->icols[s->cur];
-    k_fill_u32<<<nblocks(g.key_max + 1, 256), 256, 0, st>>>(s->cnt, 0u, g.key_max + 1);
-    k_keys<<<nb, 256, 0, st>>>(g, c + C_X * cap, c + C_Y * cap, c + C_Z * cap, s->meta,
-                               ic + I_KEY * cap, s->slot, s->cnt);
-    scan(s, SrcCount{s->cnt}, g.key_max + 1, s->cell_start, s->scan_part);
+    k_sys_prep<<<1, 1, 0, st>>>(nsys, s->sys_cnt, s->meta);
+    k_fill_u32<<<nblocks(nbuckets, 256), 256, 0, st>>>(s->cnt, 0u, nbuckets);
+    k_keys<<<nb, 256, 0, st>>>(s->sysd, nsys, ncells, c + C_X * cap, c + C_Y * cap,
+                               c + C_Z * cap, ic + I_SYS * cap, s->meta, ic + I_KEY * cap,
+                               s->slot, s->cnt);
+    scan(s, SrcCount{s->cnt}, (int)nbuckets, s->cell_start, s->scan_part);
     k_scatter<<<nb, 256, 0, st>>>(s->meta, ic + I_KEY * cap, s->slot, ic + I_REF * cap,
                                   s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref);
     k_rank<<<nb, 256, 0, st>>>(s->meta, s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref,
@@ -376,9 +452,12 @@ void enqueue_build(sph_system* s)
                                  s->icols[s->cur ^ 1], cap);
     s->cur ^= 1;
     int* ref = s->icols[s->cur] + I_REF * cap;
-    k_finalize<<<1, 1, 0, st>>>(s->meta, s->cell_start, g.key_max);
-    k_inv<<<nb, 256, 0, st>>>(s->meta, ref, s->inv);
-    k_remove_seq<<<1, 1, 0, st>>>(s->meta, ref, s->inv);
+    int* sysid = s->icols[s->cur] + I_SYS * cap;
+    k_finalize<<<nblocks(nsys, 64), 64, 0, st>>>(s->sysd, nsys, ncells, s->sys_cnt, s->meta,
+                                                  s->cell_start);
+    k_inv<<<nb, 256, 0, st>>>(s->meta, ref, sysid, s->sys_cnt, s->inv);
+    k_remove_seq<<<nblocks(nsys, 64), 64, 0, st>>>(s->meta, nsys, ncells, s->sys_cnt,
+                                                    s->cell_start, ref, s->inv);
     enqueue_tiles(s);
 }
 

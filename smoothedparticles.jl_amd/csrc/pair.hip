@@ -83,10 +83,12 @@ struct Lds64 {
     double2 xy[64], zr[64], v01[64], v2a[64];
 };
 
-__device__ inline void stage_pos64(const PairCtx& c, int q, int l, Lds64& s)
+// Candidate positions are staged with the periodic image shift of their
+// range added (zero for reference systems, where x + 0.0 == x).
+__device__ inline void stage_pos64(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
 {
-    s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
-    s.zr[l].x = COL(c, C_Z, q);
+    s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
+    s.zr[l].x = COL(c, C_Z, q) + g.sz;
 }
 
 // rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0).
@@ -104,24 +106,28 @@ struct Density64 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s) { stage_pos64(c, q, l, s); }
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    {
+        stage_pos64(c, q, l, g, s);
+    }
     static __device__ void init(Acc& a) { a.w = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds64&, int, int, int,
-                                    double, double, double, double ss, Acc& a)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds64&,
+                                    int, int, int, double, double, double, double ss, Acc& a)
     {
         const double r = sqrt(ss);
-        const double x = r * c.g.inv_h;
+        const double x = r * d.g.inv_h;
         const double u = 1.0 - x;
         const double u2 = u * u;
         a.w += u2 * u2 * (1.0 + 4.0 * x);
     }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&, Acc& a,
+                                  int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) a.w += __shfl_xor(a.w, o);
         if (!write) return;
         double v = c.overwrite ? 0.0 : COL(c, OUT, p);
-        v = v + c.ph.kW * a.w;
-        if (c.self) v = v + c.ph.kW;
+        v = v + d.ph.kW * a.w;
+        if (c.self) v = v + d.ph.kW;
         COL(c, OUT, p) = v;
     }
 };
@@ -149,50 +155,52 @@ struct Force64 {
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
-        s.zr[l] = make_double2(COL(c, C_Z, q), COL(c, C_RHO, q));
+        s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
+        s.zr[l] = make_double2(COL(c, C_Z, q) + g.sz, COL(c, C_RHO, q));
         s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
         s.v2a[l] = make_double2(COL(c, C_VZ, q), c.prho2[q]);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
-                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
-        const double x = r * c.g.inv_h;
+        const double x = r * d.g.inv_h;
         const double u = 1.0 - x;
         const double u3 = u * u * u;
         const double2 v01 = s.v01[jj], v2a = s.v2a[jj];
         double A = t.pr + v2a.y;
         const double dvx = t.vx - v01.x, dvy = t.vy - v01.y, dvz = t.vz - v2a.x;
-        if (c.ph.avk != 0.0) {
+        if (d.ph.avk != 0.0) {
             double vx = dvx * dx + dvy * dy;
             vx = vx + dvz * dz;
             if (vx < 0.0) {
                 const double rhobar = 0.5 * (t.rho + s.zr[jj].y);
-                A = A + c.ph.avk * vx / (rhobar * (r * r + c.ph.eps));
+                A = A + d.ph.avk * vx / (rhobar * (r * r + d.ph.eps));
             }
         }
         const double f = u3 * A;
         a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
-        if (c.ph.mu != 0.0) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
+        if (d.ph.mu != 0.0) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
     }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&, Acc& a,
+                                  int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) {
             a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
             a.lx += __shfl_xor(a.lx, o); a.ly += __shfl_xor(a.ly, o); a.lz += __shfl_xor(a.lz, o);
         }
         if (!write) return;
-        const double kp = -c.ph.kdW;
-        const double kl = 2.0 * c.ph.kdW * c.ph.mu / (c.ph.rho0 * c.ph.rho0);
+        const double kp = -d.ph.kdW;
+        const double kl = 2.0 * d.ph.kdW * d.ph.mu / (d.ph.rho0 * d.ph.rho0);
         double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
         double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
         double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
         ax = ax + kp * a.ax; ay = ay + kp * a.ay; az = az + kp * a.az;
-        if (c.ph.mu != 0.0) { ax = ax + kl * a.lx; ay = ay + kl * a.ly; az = az + kl * a.lz; }
+        if (d.ph.mu != 0.0) { ax = ax + kl * a.lx; ay = ay + kl * a.ly; az = az + kl * a.lz; }
         COL(c, C_AX, p) = ax; COL(c, C_AY, p) = ay; COL(c, C_AZ, p) = az;
     }
 };
@@ -211,27 +219,29 @@ struct ForceRho064 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p); t.P = COL(c, C_P, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        stage_pos64(c, q, l, s);
+        stage_pos64(c, q, l, g, s);
         s.v2a[l].y = COL(c, C_P, q);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
-                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
-        const double u = 1.0 - r * c.g.inv_h;
+        const double u = 1.0 - r * d.g.inv_h;
         const double f = u * u * u * (t.P + s.v2a[jj].y);
         a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
     }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&, Acc& a,
+                                  int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) {
             a.ax += __shfl_xor(a.ax, o); a.ay += __shfl_xor(a.ay, o); a.az += __shfl_xor(a.az, o);
         }
         if (!write) return;
-        const double k = -c.ph.kdW / (c.ph.rho0 * c.ph.rho0);
+        const double k = -d.ph.kdW / (d.ph.rho0 * d.ph.rho0);
         const double ax = c.overwrite ? 0.0 : COL(c, C_AX, p);
         const double ay = c.overwrite ? 0.0 : COL(c, C_AY, p);
         const double az = c.overwrite ? 0.0 : COL(c, C_AZ, p);
@@ -258,30 +268,32 @@ struct Continuity64 {
         t.rho = COL(c, C_RHO, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        s.xy[l] = make_double2(COL(c, C_X, q), COL(c, C_Y, q));
-        s.zr[l] = make_double2(COL(c, C_Z, q), COL(c, C_RHO, q));
+        s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
+        s.zr[l] = make_double2(COL(c, C_Z, q) + g.sz, COL(c, C_RHO, q));
         s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
         s.v2a[l].x = COL(c, C_VZ, q);
     }
     static __device__ void init(Acc& a) { a.d = 0.0; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds64& s, int jj, int,
-                                    int, double dx, double dy, double dz, double ss, Acc& a)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
-        const double u = 1.0 - r * c.g.inv_h;
+        const double u = 1.0 - r * d.g.inv_h;
         const double2 v01 = s.v01[jj];
-        double d = dx * (t.vx - v01.x) + dy * (t.vy - v01.y);
-        d = d + dz * (t.vz - s.v2a[jj].x);
-        a.d += u * u * u * (d + 2.0 * c.ph.nu * (t.rho - s.zr[jj].y));
+        double xv = dx * (t.vx - v01.x) + dy * (t.vy - v01.y);
+        xv = xv + dz * (t.vz - s.v2a[jj].x);
+        a.d += u * u * u * (xv + 2.0 * d.ph.nu * (t.rho - s.zr[jj].y));
     }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, Acc& a, int T, bool write)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&, Acc& a,
+                                  int T, bool write)
     {
         for (int o = T; o < 64; o <<= 1) a.d += __shfl_xor(a.d, o);
         if (!write) return;
         const double v = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
-        COL(c, C_DRHO, p) = v + c.ph.kdW * a.d;
+        COL(c, C_DRHO, p) = v + d.ph.kdW * a.d;
     }
 };
 
@@ -300,10 +312,13 @@ struct Pairs64 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, Lds64& s) { stage_pos64(c, q, l, s); }
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    {
+        stage_pos64(c, q, l, g, s);
+    }
     static __device__ void init(Acc&) {}
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds64&, int, int p, int q,
-                                    double, double, double, double, Acc&)
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds64&,
+                                    int, int p, int q, double, double, double, double, Acc&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
         if ((long long)k < c.pair_cap) {
@@ -311,7 +326,8 @@ struct Pairs64 {
             c.pair_out[2 * k + 1] = c.refidx[q];
         }
     }
-    static __device__ void finish(const PairCtx&, int, const Tgt&, Acc&, int, bool) {}
+    static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, Acc&, int,
+                                  bool) {}
 };
 
 template <int ROWS, bool FLAT, class Op>
@@ -320,6 +336,9 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
 {
     const int4 td = c.tiles[tile];
     const int cell = td.x, first = td.y;
+    const SysDesc& d = c.sys[c.cell_sys[cell]];
+    if (c.only_flagged && !d.pass_on) return;
+    const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
     const int cend = c.cell_start[cell + 1];
     const int lane = threadIdx.x;
@@ -332,61 +351,64 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const unsigned long long sbits = slice_pattern(S) << s;
-    const double s_thr = c.g.s_thr;
+    const double s_thr = d.g.s_thr;
     typename Op::Acc acc;
     Op::init(acc);
     unsigned npairs = 0;
     int chunk = 0;
     for (int row = 0; row < ROWS; ++row) {
-        int j0, j1;
-        if (!row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1)) continue;
-        for (int b = j0; b < j1; b += 64, ++chunk) {
-            const int nb = min(64, j1 - b);
-            if (lane < nb) Op::stage(c, b + lane, lane, lds);
-            __syncthreads();
-            unsigned long long m = 0ull;
-            if (load_masks) {
-                if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
-            } else {
-                for (int j = 0; j < nb; j += S) {
-                    const int jj = j + s;
-                    if (jj < nb) {
-                        const double2 xy = lds.xy[jj];
-                        const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                        double ss = dx * dx + dy * dy;
-                        if (!FLAT) {
-                            const double dz = tg.z - lds.zr[jj].x;
-                            ss = ss + dz * dz;
+        Seg segs[2];
+        const int nseg = row_segments<ROWS>(d, c.cell_start, lc, row, segs);
+        for (int sg = 0; sg < nseg; ++sg) {
+            const Seg seg = segs[sg];
+            for (int b = seg.j0; b < seg.j1; b += 64, ++chunk) {
+                const int nb = min(64, seg.j1 - b);
+                if (lane < nb) Op::stage(c, b + lane, lane, seg, lds);
+                __syncthreads();
+                unsigned long long m = 0ull;
+                if (load_masks) {
+                    if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
+                } else {
+                    for (int j = 0; j < nb; j += S) {
+                        const int jj = j + s;
+                        if (jj < nb) {
+                            const double2 xy = lds.xy[jj];
+                            const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                            double ss = dx * dx + dy * dy;
+                            if (!FLAT) {
+                                const double dz = tg.z - lds.zr[jj].x;
+                                ss = ss + dz * dz;
+                            }
+                            if (act && ss <= s_thr && b + jj != p) m |= 1ull << jj;
                         }
-                        if (act && ss <= s_thr && b + jj != p) m |= 1ull << jj;
+                    }
+                    if (store_masks) {
+                        unsigned long long full = m;
+                        for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
+                        if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
                     }
                 }
-                if (store_masks) {
-                    unsigned long long full = m;
-                    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
-                    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
-                }
-            }
-            npairs += (unsigned)__popcll(m);
-            if (Op::INTERACT) {
-                while (__any(m != 0ull)) {
-                    if (m) {
-                        const int jj = __ffsll((long long)m) - 1;
-                        m &= m - 1ull;
-                        const double2 xy = lds.xy[jj];
-                        const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                        const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
-                        double ss = dx * dx + dy * dy;
-                        if (!FLAT) ss = ss + dz * dz;
-                        Op::interact(c, tg, lds, jj, p, b + jj, dx, dy, dz, ss, acc);
+                npairs += (unsigned)__popcll(m);
+                if (Op::INTERACT) {
+                    while (__any(m != 0ull)) {
+                        if (m) {
+                            const int jj = __ffsll((long long)m) - 1;
+                            m &= m - 1ull;
+                            const double2 xy = lds.xy[jj];
+                            const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                            const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
+                            double ss = dx * dx + dy * dy;
+                            if (!FLAT) ss = ss + dz * dz;
+                            Op::interact(c, d, tg, lds, jj, p, b + jj, dx, dy, dz, ss, acc);
+                        }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     if (c.pair_counter) count_pairs(c, npairs);
-    Op::finish(c, p, tg, acc, T, act && s == 0);
+    Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
 
 template <int ROWS, bool FLAT, class Op>
@@ -430,33 +452,34 @@ struct Density32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds32& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
+                                 double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
-                             (float)(COL(c, C_Z, q) - oz), 0.0f);
+        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
+                             (float)((COL(c, C_Y, q) + g.sy) - oy),
+                             (float)((COL(c, C_Z, q) + g.sz) - oz), 0.0f);
     }
     static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds32&, int, int, int,
-                                    float, float, float, float ss, Part& q)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds32&,
+                                    int, int, int, float, float, float, float ss, Part& q)
     {
         const float r = __builtin_sqrtf(ss);
-        const float x = r * (float)c.g.inv_h;
+        const float x = r * (float)d.g.inv_h;
         const float u = 1.0f - x;
         const float u2 = u * u;
         q.w = fmaf(u2 * u2, fmaf(4.0f, x, 1.0f), q.w);
     }
     static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.w); }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
-                                  bool write, int out_col)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
+                                  const Acc& a, int T, bool write, int out_col)
     {
         double w = (double)a.s + (double)a.e;
         for (int o = T; o < 64; o <<= 1) w += __shfl_xor(w, o);
         if (!write) return;
         double v = c.overwrite ? 0.0 : COL(c, out_col, p);
-        v = v + c.ph.kW * w;
-        if (c.self) v = v + c.ph.kW;
+        v = v + d.ph.kW * w;
+        if (c.self) v = v + d.ph.kW;
         COL(c, out_col, p) = v;
     }
 };
@@ -484,11 +507,12 @@ struct Force32 {
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
-    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds32& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
+                                 double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
-                             (float)(COL(c, C_Z, q) - oz), (float)c.prho2[q]);
+        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
+                             (float)((COL(c, C_Y, q) + g.sy) - oy),
+                             (float)((COL(c, C_Z, q) + g.sz) - oz), (float)c.prho2[q]);
         s.v[l] = make_float4((float)COL(c, C_VX, q), (float)COL(c, C_VY, q),
                              (float)COL(c, C_VZ, q), (float)COL(c, C_RHO, q));
     }
@@ -498,26 +522,27 @@ struct Force32 {
         a.lx = a.ly = a.lz = a.fx = a.fy = a.fz = 0.0f;
     }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
-    static __device__ void interact(const PairCtx& c, const Tgt& t, const Lds32& s, int jj, int, int,
-                                    float dx, float dy, float dz, float ss, Part& q)
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds32& s, int jj, int, int, float dx, float dy,
+                                    float dz, float ss, Part& q)
     {
         const float r = __builtin_sqrtf(ss);
-        const float u = 1.0f - r * (float)c.g.inv_h;
+        const float u = 1.0f - r * (float)d.g.inv_h;
         const float u3 = u * u * u;
         const float4 a = s.a[jj], v = s.v[jj];
         float A = t.pr + a.w;
         const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
-        if (c.ph.avk != 0.0) {
+        if (d.ph.avk != 0.0) {
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
             if (vx < 0.0f) {
                 const float rhobar = 0.5f * (t.rho + v.w);
-                const float den = rhobar * (ss + (float)c.ph.eps);
-                A = fmaf((float)c.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
+                const float den = rhobar * (ss + (float)d.ph.eps);
+                A = fmaf((float)d.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
             }
         }
         const float f = u3 * A;
         q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
-        if (c.ph.mu != 0.0) {
+        if (d.ph.mu != 0.0) {
             q.lx = fmaf(u3, dvx, q.lx); q.ly = fmaf(u3, dvy, q.ly); q.lz = fmaf(u3, dvz, q.lz);
         }
     }
@@ -530,8 +555,8 @@ struct Force32 {
         two_sum_add(a.ly, a.fy, q.ly);
         two_sum_add(a.lz, a.fz, q.lz);
     }
-    static __device__ void finish(const PairCtx& c, int p, const Tgt&, const Acc& a, int T,
-                                  bool write, int)
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
+                                  const Acc& a, int T, bool write, int)
     {
         double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
                az = (double)a.sz + (double)a.ez;
@@ -542,13 +567,13 @@ struct Force32 {
             lx += __shfl_xor(lx, o); ly += __shfl_xor(ly, o); lz += __shfl_xor(lz, o);
         }
         if (!write) return;
-        const double kp = -c.ph.kdW;
-        const double kl = 2.0 * c.ph.kdW * c.ph.mu / (c.ph.rho0 * c.ph.rho0);
+        const double kp = -d.ph.kdW;
+        const double kl = 2.0 * d.ph.kdW * d.ph.mu / (d.ph.rho0 * d.ph.rho0);
         double ox = c.overwrite ? 0.0 : COL(c, C_AX, p);
         double oy = c.overwrite ? 0.0 : COL(c, C_AY, p);
         double oz = c.overwrite ? 0.0 : COL(c, C_AZ, p);
         ox = ox + kp * ax; oy = oy + kp * ay; oz = oz + kp * az;
-        if (c.ph.mu != 0.0) { ox = ox + kl * lx; oy = oy + kl * ly; oz = oz + kl * lz; }
+        if (d.ph.mu != 0.0) { ox = ox + kl * lx; oy = oy + kl * ly; oz = oz + kl * lz; }
         COL(c, C_AX, p) = ox; COL(c, C_AY, p) = oy; COL(c, C_AZ, p) = oz;
     }
 };
@@ -572,16 +597,17 @@ struct Pairs32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    static __device__ void stage(const PairCtx& c, int q, int l, double ox, double oy, double oz,
-                                 Lds32& s)
+    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
+                                 double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)(COL(c, C_X, q) - ox), (float)(COL(c, C_Y, q) - oy),
-                             (float)(COL(c, C_Z, q) - oz), 0.0f);
+        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
+                             (float)((COL(c, C_Y, q) + g.sy) - oy),
+                             (float)((COL(c, C_Z, q) + g.sz) - oz), 0.0f);
     }
     static __device__ void init(Acc&) {}
     static __device__ void zero(Part&) {}
-    static __device__ void interact(const PairCtx& c, const Tgt&, const Lds32&, int, int p, int q,
-                                    float, float, float, float, Part&)
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds32&,
+                                    int, int p, int q, float, float, float, float, Part&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
         if ((long long)k < c.pair_cap) {
@@ -590,7 +616,8 @@ struct Pairs32 {
         }
     }
     static __device__ void fold(Acc&, const Part&) {}
-    static __device__ void finish(const PairCtx&, int, const Tgt&, const Acc&, int, bool, int) {}
+    static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, const Acc&,
+                                  int, bool, int) {}
 };
 
 template <int ROWS, bool FLAT, class Op>
@@ -599,6 +626,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
 {
     const int4 td = c.tiles[tile];
     const int cell = td.x, first = td.y;
+    const SysDesc& d = c.sys[c.cell_sys[cell]];
+    if (c.only_flagged && !d.pass_on) return;
+    const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
     const int cend = c.cell_start[cell + 1];
     const int lane = threadIdx.x;
@@ -613,7 +643,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
-    const double s_thr = c.g.s_thr;
+    const double s_thr = d.g.s_thr;
     const float thr_lo = (float)(s_thr * (1.0 - BAND));
     const float thr_hi = (float)(s_thr * (1.0 + BAND));
     const unsigned long long sbits = slice_pattern(S) << s;
@@ -622,70 +652,75 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     unsigned npairs = 0;
     int chunk = 0;
     for (int row = 0; row < ROWS; ++row) {
-        int j0, j1;
-        if (!row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1)) continue;
-        for (int b = j0; b < j1; b += 64, ++chunk) {
-            const int nb = min(64, j1 - b);
-            if (lane < nb) Op::stage(c, b + lane, lane, ox, oy, oz, lds);
-            __syncthreads();
-            unsigned long long m = 0ull;
-            if (load_masks) {
-                if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
-            } else {
-                for (int j = 0; j < nb; j += S) {
-                    const int jj = j + s;
-                    const bool in = jj < nb;
-                    const int jr = in ? jj : 0;
-                    const float4 a = lds.a[jr];
-                    const float dx = tg.x - a.x, dy = tg.y - a.y;
-                    const float dz = FLAT ? 0.0f : tg.z - a.z;
-                    const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const int q = b + jr;
-                    bool ok = ss <= thr_lo;
-                    const bool band = in && act && ss > thr_lo && ss <= thr_hi;
-                    if (__any(band)) {
-                        if (band) {
-                            // exact FP64 predicate of the reference
-                            const double e1 = px - COL(c, C_X, q), e2 = py - COL(c, C_Y, q);
-                            double es = e1 * e1 + e2 * e2;
-                            if (!FLAT) {
-                                const double e3 = pz - COL(c, C_Z, q);
-                                es = es + e3 * e3;
-                            }
-                            ok = es <= s_thr;
-                        }
-                    }
-                    if (in && act && ok && q != p) m |= 1ull << jj;
-                }
-                if (store_masks) {
-                    unsigned long long full = m;
-                    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
-                    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
-                }
-            }
-            npairs += (unsigned)__popcll(m);
-            if (Op::INTERACT) {
-                typename Op::Part part;
-                Op::zero(part);
-                while (__any(m != 0ull)) {
-                    if (m) {
-                        const int jj = __ffsll((long long)m) - 1;
-                        m &= m - 1ull;
-                        const float4 a = lds.a[jj];
+        Seg segs[2];
+        const int nseg = row_segments<ROWS>(d, c.cell_start, lc, row, segs);
+        for (int sg = 0; sg < nseg; ++sg) {
+            const Seg seg = segs[sg];
+            for (int b = seg.j0; b < seg.j1; b += 64, ++chunk) {
+                const int nb = min(64, seg.j1 - b);
+                if (lane < nb) Op::stage(c, b + lane, lane, seg, ox, oy, oz, lds);
+                __syncthreads();
+                unsigned long long m = 0ull;
+                if (load_masks) {
+                    if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
+                } else {
+                    for (int j = 0; j < nb; j += S) {
+                        const int jj = j + s;
+                        const bool in = jj < nb;
+                        const int jr = in ? jj : 0;
+                        const float4 a = lds.a[jr];
                         const float dx = tg.x - a.x, dy = tg.y - a.y;
                         const float dz = FLAT ? 0.0f : tg.z - a.z;
                         const float ss =
                             FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        Op::interact(c, tg, lds, jj, p, b + jj, dx, dy, dz, ss, part);
+                        const int q = b + jr;
+                        bool ok = ss <= thr_lo;
+                        const bool band = in && act && ss > thr_lo && ss <= thr_hi;
+                        if (__any(band)) {
+                            if (band) {
+                                // exact FP64 predicate of the reference
+                                const double e1 = px - (COL(c, C_X, q) + seg.sx);
+                                const double e2 = py - (COL(c, C_Y, q) + seg.sy);
+                                double es = e1 * e1 + e2 * e2;
+                                if (!FLAT) {
+                                    const double e3 = pz - (COL(c, C_Z, q) + seg.sz);
+                                    es = es + e3 * e3;
+                                }
+                                ok = es <= s_thr;
+                            }
+                        }
+                        if (in && act && ok && q != p) m |= 1ull << jj;
+                    }
+                    if (store_masks) {
+                        unsigned long long full = m;
+                        for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
+                        if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
                     }
                 }
-                Op::fold(acc, part);
+                npairs += (unsigned)__popcll(m);
+                if (Op::INTERACT) {
+                    typename Op::Part part;
+                    Op::zero(part);
+                    while (__any(m != 0ull)) {
+                        if (m) {
+                            const int jj = __ffsll((long long)m) - 1;
+                            m &= m - 1ull;
+                            const float4 a = lds.a[jj];
+                            const float dx = tg.x - a.x, dy = tg.y - a.y;
+                            const float dz = FLAT ? 0.0f : tg.z - a.z;
+                            const float ss =
+                                FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            Op::interact(c, d, tg, lds, jj, p, b + jj, dx, dy, dz, ss, part);
+                        }
+                    }
+                    Op::fold(acc, part);
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     if (c.pair_counter) count_pairs(c, npairs);
-    Op::finish(c, p, tg, acc, T, act && s == 0, out_col);
+    Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
 template <int ROWS, bool FLAT, class Op>
@@ -713,10 +748,11 @@ __global__ __launch_bounds__(256) void k_candidates(PairCtx c, const int* __rest
     unsigned long long all = 0, fluid = 0;
     if (p < n) {
         const int cell = key[p];
+        const SysDesc& d = c.sys[c.cell_sys[cell]];
         for (int row = 0; row < ROWS; ++row) {
-            int j0, j1;
-            if (row_range<ROWS>(c.g, c.cell_start, cell, row, j0, j1))
-                all += (unsigned long long)(j1 - j0);
+            Seg segs[2];
+            const int ns = row_segments<ROWS>(d, c.cell_start, cell - d.cell_base, row, segs);
+            for (int q = 0; q < ns; ++q) all += (unsigned long long)(segs[q].j1 - segs[q].j0);
         }
         if (COL(c, C_TYPE, p) == 0.0) fluid = all;
     }
@@ -731,12 +767,12 @@ __global__ __launch_bounds__(256) void k_candidates(PairCtx c, const int* __rest
     }
 }
 
-void launch_candidates(const PairCtx& c, const int* key, unsigned long long* out, long long n,
-                       hipStream_t st)
+void launch_candidates(int rows, const PairCtx& c, const int* key, unsigned long long* out,
+                       long long n, hipStream_t st)
 {
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (nb == 0) return;
-    if (c.g.dim == 2) k_candidates<3><<<nb, 256, 0, st>>>(c, key, out);
+    if (rows == 3) k_candidates<3><<<nb, 256, 0, st>>>(c, key, out);
     else k_candidates<9><<<nb, 256, 0, st>>>(c, key, out);
 }
 
@@ -769,11 +805,12 @@ static void launch_rows(int op, int precision, const PairCtx& c, int grid, hipSt
     }
 }
 
-void launch_pair(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
+void launch_pair(int op, int precision, int rows, int flat, const PairCtx& c, int grid,
+                 hipStream_t st)
 {
     if (grid <= 0) return;
-    if (c.g.dim == 2) {
-        if (c.g.flat) launch_rows<3, true>(op, precision, c, grid, st);
+    if (rows == 3) {
+        if (flat) launch_rows<3, true>(op, precision, c, grid, st);
         else launch_rows<3, false>(op, precision, c, grid, st);
     } else {
         launch_rows<9, false>(op, precision, c, grid, st);

@@ -1,12 +1,23 @@
 // Internal definitions shared by the HIP translation units of libsph_hip.
 //
-// Device layout (one system):
+// One handle (struct sph_system) holds a batch of independent particle
+// systems packed into one set of device arrays; the single-system API of
+// sph.h is a batch of one.
+//
+// Device layout:
 //   cols[buf]   NCOL FP64 columns of `cap` entries each, in cell-sorted order:
 //               x y z | vx vy vz | ax ay az | rho P Drho rho0 type
-//   icols[buf]  refidx (reference index of the particle, src/core.jl order)
-//               and key (0-based linear cell key, key_max = removed)
+//   icols[buf]  refidx (index of the particle in its system's reference
+//               order, src/core.jl), key (global cell key) and sys (system id)
 //   Two buffers: the cell build gathers buffer `cur` into `cur^1` and flips.
+//   sysd        one SysDesc per system: cell geometry, parameters, the
+//               system's range of global cell keys
 //   meta        small int32 counters read by kernels (particle count, tiles)
+//
+// Global cell keys: system s owns keys [cell_base_s, cell_base_s + key_max_s);
+// after all cells come one removal bucket per system and one bucket for
+// particles of retired systems.  Sorting by global key therefore keeps every
+// system's particles contiguous and cell-sorted.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,10 +33,10 @@ enum Col {
     C_X = 0, C_Y, C_Z, C_VX, C_VY, C_VZ, C_AX, C_AY, C_AZ,
     C_RHO, C_P, C_DRHO, C_RHO0, C_TYPE, NCOL
 };
-enum ICol { I_REF = 0, I_KEY, NICOL };
+enum ICol { I_REF = 0, I_KEY, I_SYS, NICOL };
 
 enum Meta {
-    META_N = 0,       // active particle count
+    META_N = 0,       // active particle count (all systems)
     META_NTOT = 1,    // count at the start of the last build
     META_NREM = 2,    // particles removed by the last build
     META_NTILES = 3,  // tiles of the pair kernels
@@ -35,10 +46,13 @@ enum Meta {
     META_SLOTS = 8
 };
 
+// Per-system counters on the device (SYSCNT_* ints per system).
+enum SysCnt { SC_NTOT = 0, SC_N = 1, SC_NREM = 2, SC_OFF = 3, SC_REFBASE = 4, SYSCNT = 5 };
+
 // Cell geometry of the reference (src/structs.jl:63-82) plus the exact
 // squared-distance threshold of the pair predicate.
 struct Geo {
-    double h;        // neighbour radius = cell size
+    double h;        // neighbour radius
     double inv_h;
     double s_thr;    // largest s with RN(sqrt(s)) <= h (src/core.jl:105)
     double bmin[3], bmax[3];
@@ -47,6 +61,10 @@ struct Geo {
     int key_max;
     int dim;         // 2 when key_lim[3] == 1
     int flat;        // bmin[2] == bmax[2]: every particle has the same z
+    int periodic;    // 1: periodic in x and y (and z in 3D), minimum image
+    int pad;
+    double box[3];   // periodic box lengths (bmax - bmin)
+    double cell[3];  // periodic cell lengths box / L (>= h)
 };
 
 // Parameters of the pair and unary kernels (derived from sph_params).
@@ -60,6 +78,16 @@ struct Phys {
     double eps;   // 0.01 * hs^2
 };
 
+// One system of a batch.
+struct SysDesc {
+    Geo g;
+    Phys ph;
+    int cell_base;   // first global cell key
+    int live;        // 0 once retired: its particles go to the dead bucket
+    int pass_on;     // 1: passes with only_flagged set write this system
+    int pad;
+};
+
 // Pair-mask modes of the pair kernels.  A mask is the 64-bit set of
 // accepted candidates of one target in one 64-candidate chunk; the density
 // pass records them and the passes after it on the same cell list read
@@ -67,11 +95,11 @@ struct Phys {
 enum MaskMode { MASK_TEST = 0, MASK_TEST_STORE = 1, MASK_LOAD = 2 };
 
 struct PairCtx {
-    Geo g;
-    Phys ph;
+    const SysDesc* sys;       // device array, one per system
+    const int* cell_sys;      // global cell -> system
     int T;                    // targets per tile (8..64, power of two)
     int cap;
-    const int4* tiles;        // {cell, first target, mask base lo, mask base hi}
+    const int4* tiles;        // {global cell, first target, mask base lo, mask base hi}
     unsigned long long* masks;
     int mask_mode;
     const int* cell_start;
@@ -86,6 +114,7 @@ struct PairCtx {
     long long pair_cap;
     int self;                 // add the self term (density)
     int overwrite;            // write instead of accumulate
+    int only_flagged;         // skip systems whose pass_on is 0
 };
 
 // Binary operator ids inside the library.
@@ -100,43 +129,108 @@ enum PairOp {
     NPAIROP
 };
 
-// Row r of the 3 (2D) or 9 (3D) stencil rows of `cell`: the cells
-// key + L1*dj + L1*L2*dk + di for di = -1..1 (src/structs.jl:70-82) are
-// consecutive, so a row is one contiguous particle range [j0, j1) after the
-// linear bounds test 1 <= key <= key_max of src/core.jl:98.
+// One contiguous candidate range of a stencil row, with the periodic image
+// shift added to the candidates' coordinates.
+struct Seg {
+    int j0, j1;
+    double sx, sy, sz;
+};
+
+// The stencil rows of local cell `lc` of system `d`.
+//
+// Reference systems: row r = (dj, dk) holds the cells
+// key + L1*dj + L1*L2*dk + di for di = -1..1 (src/structs.jl:70-82), which
+// are consecutive, so after the linear bounds test 1 <= key <= key_max of
+// src/core.jl:98 a row is one contiguous particle range; rows wrap and
+// small grids alias exactly as in the reference.
+//
+// Periodic systems (not in the reference): rows wrap per axis with the
+// minimum-image shift; a row splits into at most two ranges at the x seam.
+// Every periodic axis has at least 3 cells, so no cell is visited twice.
+// Returns the number of ranges (0..2) written to seg.
 template <int ROWS>
-__device__ inline bool row_range(const Geo& g, const int* cell_start, int cell, int row, int& j0,
-                                 int& j1)
+__device__ inline int row_segments(const SysDesc& d, const int* cell_start, int lc, int row,
+                                   Seg* seg)
 {
+    const Geo& g = d.g;
     const int dj = row % 3 - 1;
     const int dk = ROWS == 3 ? 0 : row / 3 - 1;
-    const long long mid = (long long)cell + (long long)g.L1 * dj + (long long)g.L1 * g.L2 * dk;
-    long long lo = mid - 1, hi = mid + 1;
-    if (lo < 0) lo = 0;
-    if (hi > g.key_max - 1) hi = g.key_max - 1;
-    if (lo > hi) return false;
-    j0 = cell_start[lo];
-    j1 = cell_start[hi + 1];
-    return j0 < j1;
+    const int base = d.cell_base;
+    if (!g.periodic) {
+        const long long mid = (long long)lc + (long long)g.L1 * dj + (long long)g.L1 * g.L2 * dk;
+        long long lo = mid - 1, hi = mid + 1;
+        if (lo < 0) lo = 0;
+        if (hi > g.key_max - 1) hi = g.key_max - 1;
+        if (lo > hi) return 0;
+        seg[0].j0 = cell_start[base + lo];
+        seg[0].j1 = cell_start[base + hi + 1];
+        seg[0].sx = seg[0].sy = seg[0].sz = 0.0;
+        return seg[0].j0 < seg[0].j1 ? 1 : 0;
+    }
+    const int ci = lc % g.L1;
+    const int cj = (lc / g.L1) % g.L2;
+    const int ck = lc / (g.L1 * g.L2);
+    int j = cj + dj, k = ck + dk;
+    double sy = 0.0, sz = 0.0;
+    if (j < 0) { j += g.L2; sy = -g.box[1]; }
+    else if (j >= g.L2) { j -= g.L2; sy = g.box[1]; }
+    if (ROWS == 9) {
+        if (k < 0) { k += g.L3; sz = -g.box[2]; }
+        else if (k >= g.L3) { k -= g.L3; sz = g.box[2]; }
+    }
+    const int rowbase = base + g.L1 * (j + g.L2 * k);
+    int n = 0;
+    int lo = ci - 1;
+    const int hi = ci + 1;
+    if (lo < 0) {  // cell L1-1 seen through the left seam
+        const int a = cell_start[rowbase + g.L1 - 1], b = cell_start[rowbase + g.L1];
+        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = -g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
+        lo = 0;
+    }
+    const int hi_in = hi >= g.L1 ? g.L1 - 1 : hi;
+    {
+        const int a = cell_start[rowbase + lo], b = cell_start[rowbase + hi_in + 1];
+        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = 0.0; seg[n].sy = sy; seg[n].sz = sz; ++n; }
+    }
+    if (hi >= g.L1) {  // cell 0 seen through the right seam
+        const int a = cell_start[rowbase], b = cell_start[rowbase + 1];
+        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
+    }
+    return n;
 }
 
 // Kernel launchers (implemented in the .hip files).
-void launch_pair(int op, int precision, const PairCtx& c, int grid, hipStream_t st);
-void launch_candidates(const PairCtx& c, const int* key, unsigned long long* out, long long n,
-                       hipStream_t st);
+void launch_pair(int op, int precision, int rows, int flat, const PairCtx& c, int grid,
+                 hipStream_t st);
+void launch_candidates(int rows, const PairCtx& c, const int* key, unsigned long long* out,
+                       long long n, hipStream_t st);
 
 }  // namespace sph
 
-// The opaque handle of sph.h.
+// The opaque handle of sph.h / sph_batch.h: a batch of systems.
 struct sph_system {
-    sph::Geo geo;
     int precision;
     int tile_T;          // current targets per tile
     int tile_auto;       // choose tile_T from occupancy at each build
+    int rows;            // 3 (2D) or 9 (3D): the same for every system of a batch
+    int flat;            // every system is flat 2D
     long long n;         // host mirror of meta[META_N]
     long long cap;
     bool cells_valid;
     hipStream_t stream;
+
+    // systems
+    std::vector<sph::SysDesc> sys_host;
+    std::vector<long long> sys_n;      // particles per system (host mirror)
+    std::vector<long long> sys_off;    // first particle of each system (valid with cells)
+    sph::SysDesc* sysd;                // device copy
+    long long sysd_cap;
+    int* cell_sys;                     // global cell -> system
+    long long cells_total;             // sum of key_max of all systems
+    long long cells_cap;               // allocation of the per-cell arrays
+    int* sys_cnt;                      // device: SYSCNT ints per system
+    int* sys_cnt_host;                 // pinned mirror
+    long long sys_cnt_cap;
 
     double* cols[2];
     int* icols[2];
@@ -147,9 +241,9 @@ struct sph_system {
     unsigned long long* pair_counter;
 
     // build scratch
-    unsigned* cnt;       // key_max + 1
-    int* cell_start;     // key_max + 2
-    int* tile_start;     // key_max + 2
+    unsigned* cnt;       // buckets
+    int* cell_start;     // buckets + 1
+    int* tile_start;     // cells + 1
     unsigned* scan_part; // scan partials
     long long scan_part_n;
     int* slot;           // cap
@@ -160,8 +254,8 @@ struct sph_system {
     int* inv;            // cap
     int4* tiles;         // tile bound
     long long tile_cap;
-    int* cell_chunks;    // key_max: 64-candidate chunks of a target of the cell
-    unsigned long long* mask_start;  // key_max + 2: first mask word of each cell
+    int* cell_chunks;    // cells: 64-candidate chunks of a target of the cell
+    unsigned long long* mask_start;  // cells + 1: first mask word of each cell
     unsigned long long* scan_part64;
     unsigned long long* masks;       // pair masks of the current cell list
     long long mask_cap;              // words allocated
@@ -169,11 +263,18 @@ struct sph_system {
     bool masks_valid;                // masks match the cell list and positions
     bool use_masks;                  // option: reuse masks across passes
     double* prho2;       // cap
-    double* stage;       // 3 * cap (field IO)
+    double* stage;       // 9 * cap (field IO)
     double* red_part;    // reduction partials
     long long red_part_n;
     double* red_out;     // 3 reduction results
     unsigned long long step_pairs[2];  // pair counts of the last sph_step
+
+    // batch bookkeeping (host only)
+    std::vector<int> slot_of_id;       // stable system id -> slot (-1 once compacted out)
+    std::vector<int> id_of_slot;       // slot -> stable id
+    std::vector<char> rho0_pending;    // slot: reference density not yet initialised
+    bool retired_since_build;          // retired particles are still in their cells
+    long long bytes;                   // device bytes held by the handle
 
     // profiling
     bool prof;

@@ -13,11 +13,13 @@ namespace sph {
 
 // ------------------------------------------------------------ operators --
 __global__ __launch_bounds__(256) void k_unary(int op, const int* meta, double* cols,
-                                               long long cap, Phys ph, double* prho2)
+                                               long long cap, const SysDesc* __restrict__ sysd,
+                                               const int* __restrict__ sysid, double* prho2)
 {
     const int n = meta[META_N];
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    const Phys& ph = sysd[sysid[i]].ph;
     switch (op) {
     case SPH_OP_RESET_RHO:  // tests/test_collision_2d.jl:84-86
         UCOL(cols, cap, C_RHO, i) = 0.0;
@@ -85,12 +87,14 @@ __global__ __launch_bounds__(256) void k_prho2(const int* meta, const double* co
 // v += dt (a + g); x += dt v  (examples/collapse_dry.jl:148-159 with a
 // full step instead of two half steps).
 __global__ __launch_bounds__(256) void k_kick_drift(const int* meta, double* cols, long long cap,
-                                                    Phys ph)
+                                                    const SysDesc* __restrict__ sysd,
+                                                    const int* __restrict__ sysid)
 {
     const int n = meta[META_N];
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     if (UCOL(cols, cap, C_TYPE, i) != 0.0) return;
+    const Phys& ph = sysd[sysid[i]].ph;
     const double k = ph.coeff * ph.dt;
     for (int d = 0; d < 3; ++d) {
         const double v = UCOL(cols, cap, C_VX + d, i) + k * (UCOL(cols, cap, C_AX + d, i) + ph.g[d]);
@@ -156,16 +160,19 @@ __device__ inline double energy_of(int what, const double* cols, long long cap, 
     return 0.5 * ph.m * vv + 0.5 * ph.m * (ph.c * ph.c) * (d * d) / (ph.rho0 * ph.rho0);
 }
 
-__global__ __launch_bounds__(256) void k_reduce_part(int what, const int* meta,
-                                                     const double* cols, long long cap, Phys ph,
+// Sum over the particles [off, off + n) of one system.
+__global__ __launch_bounds__(256) void k_reduce_part(int what, int off, int n,
+                                                     const double* cols, long long cap,
+                                                     const SysDesc* __restrict__ sysd, int sys,
                                                      double* part)
 {
-    const int n = meta[META_N];
-    const int base = blockIdx.x * 256 * RED_ITEMS;
+    const Phys& ph = sysd[sys].ph;
+    const int base = off + blockIdx.x * 256 * RED_ITEMS;
+    const int end = off + n;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int it = 0; it < RED_ITEMS; ++it) {
         const int i = base + it * 256 + threadIdx.x;
-        if (i >= n) break;
+        if (i >= end) break;
         if (what == SPH_REDUCE_MOMENTUM) {
             a0 += ph.m * UCOL(cols, cap, C_VX, i);
             a1 += ph.m * UCOL(cols, cap, C_VY, i);
@@ -196,33 +203,32 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nb
 }
 
 // ------------------------------------------------------------ field I/O --
-// Host order is the reference order: host slot refidx[k] <-> device slot k.
-__global__ __launch_bounds__(256) void k_field_out(const int* meta, const int* __restrict__ ref,
+// Host order is the system's reference order: host slot refidx[k] <->
+// device slot k, for k in the system's range [off, off + n).
+__global__ __launch_bounds__(256) void k_field_out(int off, int n, const int* __restrict__ ref,
                                                    const double* cols, long long cap, int col0,
                                                    int ncomp, double* out)
 {
-    const int n = meta[META_N];
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
+    const int k = off + blockIdx.x * 256 + threadIdx.x;
+    if (k >= off + n) return;
     const long long r = ref[k];
     for (int d = 0; d < ncomp; ++d) out[ncomp * r + d] = UCOL(cols, cap, col0 + d, k);
 }
 
-__global__ __launch_bounds__(256) void k_field_in(const int* meta, const int* __restrict__ ref,
+__global__ __launch_bounds__(256) void k_field_in(int off, int n, const int* __restrict__ ref,
                                                   double* cols, long long cap, int col0, int ncomp,
                                                   const double* in)
 {
-    const int n = meta[META_N];
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
+    const int k = off + blockIdx.x * 256 + threadIdx.x;
+    if (k >= off + n) return;
     const long long r = ref[k];
     for (int d = 0; d < ncomp; ++d) UCOL(cols, cap, col0 + d, k) = in[ncomp * r + d];
 }
 
 // Append m particles at device slots [n0, n0+m) with reference indices
 // n0.. ; `in` holds x[m][3], v[m][3], rho[m], P[m], type[m] back to back.
-__global__ __launch_bounds__(256) void k_append(int n0, int m, const double* in, int has_v,
-                                                int has_rho, int has_P, int has_type,
+__global__ __launch_bounds__(256) void k_append(int n0, int m, int ref0, int sys, const double* in,
+                                                int has_v, int has_rho, int has_P, int has_type,
                                                 double* cols, int* icols, long long cap)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -241,19 +247,40 @@ __global__ __launch_bounds__(256) void k_append(int n0, int m, const double* in,
     if (has_rho) UCOL(cols, cap, C_RHO, k) = rho[i];
     if (has_P) UCOL(cols, cap, C_P, k) = P[i];
     if (has_type) UCOL(cols, cap, C_TYPE, k) = type[i];
-    icols[I_REF * cap + k] = k;
+    icols[I_REF * cap + k] = ref0 + i;
     icols[I_KEY * cap + k] = 0;
+    icols[I_SYS * cap + k] = sys;
 }
 
 __global__ void k_set_meta(int* meta, int idx, int v) { meta[idx] = v; }
 
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+// old slot -> new slot of the system column after retired systems are
+// compacted out of the table (map has one entry per old slot).
+__global__ __launch_bounds__(256) void k_remap_sys(const int* meta, int* sysid, const int* map)
+{
+    const int n = meta[META_N];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) sysid[k] = map[sysid[k]];
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* p, int v, long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // ------------------------------------------------------------ launchers --
 static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
-void enqueue_unary(sph_system* s, int op, const Phys& ph, double* prho2)
+static inline const int* sysid_col(sph_system* s) { return s->icols[s->cur] + I_SYS * s->cap; }
+
+void enqueue_unary(sph_system* s, int op, double* prho2)
 {
     if (s->n <= 0) return;
-    k_unary<<<nblk(s->n), 256, 0, s->stream>>>(op, s->meta, s->cols[s->cur], s->cap, ph, prho2);
+    k_unary<<<nblk(s->n), 256, 0, s->stream>>>(op, s->meta, s->cols[s->cur], s->cap, s->sysd,
+                                              sysid_col(s), prho2);
 }
 
 void enqueue_prho2(sph_system* s)
@@ -262,44 +289,61 @@ void enqueue_prho2(sph_system* s)
     k_prho2<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, s->prho2);
 }
 
-void enqueue_kick_drift(sph_system* s, const Phys& ph)
+void enqueue_kick_drift(sph_system* s)
 {
     if (s->n <= 0) return;
-    k_kick_drift<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, ph);
+    k_kick_drift<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, s->sysd,
+                                                   sysid_col(s));
 }
 
 long long reduce_partials_needed(long long n) { return (n + 256 * RED_ITEMS - 1) / (256 * RED_ITEMS) + 1; }
 
-void enqueue_reduce(sph_system* s, int what, const Phys& ph, double* out_dev)
+void enqueue_reduce(sph_system* s, int sys, long long off, long long n, int what, double* out_dev)
 {
-    const int nb = (int)((s->n + 256 * RED_ITEMS - 1) / (256 * RED_ITEMS));
+    const int nb = (int)((n + 256 * RED_ITEMS - 1) / (256 * RED_ITEMS));
     if (nb > 0)
-        k_reduce_part<<<nb, 256, 0, s->stream>>>(what, s->meta, s->cols[s->cur], s->cap, ph,
-                                                 s->red_part);
+        k_reduce_part<<<nb, 256, 0, s->stream>>>(what, (int)off, (int)n, s->cols[s->cur], s->cap,
+                                                 s->sysd, sys, s->red_part);
     k_reduce_final<<<1, 256, 0, s->stream>>>(s->red_part, nb, out_dev);
 }
 
-void enqueue_field_out(sph_system* s, int col0, int ncomp, double* out_dev)
+void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,
+                       double* out_dev)
 {
-    if (s->n <= 0) return;
-    k_field_out<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->icols[s->cur] + I_REF * s->cap,
-                                                  s->cols[s->cur], s->cap, col0, ncomp, out_dev);
+    if (n <= 0) return;
+    k_field_out<<<nblk(n), 256, 0, s->stream>>>((int)off, (int)n, s->icols[s->cur] + I_REF * s->cap,
+                                               s->cols[s->cur], s->cap, col0, ncomp, out_dev);
 }
 
-void enqueue_field_in(sph_system* s, int col0, int ncomp, const double* in_dev)
+void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,
+                      const double* in_dev)
 {
-    if (s->n <= 0) return;
-    k_field_in<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->icols[s->cur] + I_REF * s->cap,
-                                                 s->cols[s->cur], s->cap, col0, ncomp, in_dev);
+    if (n <= 0) return;
+    k_field_in<<<nblk(n), 256, 0, s->stream>>>((int)off, (int)n, s->icols[s->cur] + I_REF * s->cap,
+                                              s->cols[s->cur], s->cap, col0, ncomp, in_dev);
 }
 
-void enqueue_append(sph_system* s, int n0, int m, const double* in_dev, int has_v, int has_rho,
-                    int has_P, int has_type)
+void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const double* in_dev,
+                    int has_v, int has_rho, int has_P, int has_type)
 {
     if (m <= 0) return;
-    k_append<<<nblk(m), 256, 0, s->stream>>>(n0, m, in_dev, has_v, has_rho, has_P, has_type,
-                                            s->cols[s->cur], s->icols[s->cur], s->cap);
+    k_append<<<nblk(m), 256, 0, s->stream>>>(n0, m, ref0, sys, in_dev, has_v, has_rho, has_P,
+                                            has_type, s->cols[s->cur], s->icols[s->cur], s->cap);
     k_set_meta<<<1, 1, 0, s->stream>>>(s->meta, META_N, n0 + m);
+}
+
+void enqueue_set_int(sph_system* s, int* p, int v) { k_set_int<<<1, 1, 0, s->stream>>>(p, v); }
+
+void enqueue_fill_int(sph_system* s, int* p, int v, long long n)
+{
+    if (n > 0) k_fill_int<<<nblk(n), 256, 0, s->stream>>>(p, v, n);
+}
+
+void enqueue_remap_sys(sph_system* s, const int* map_dev)
+{
+    if (s->n <= 0) return;
+    k_remap_sys<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->icols[s->cur] + I_SYS * s->cap,
+                                                  map_dev);
 }
 
 }  // namespace sph

@@ -12,6 +12,9 @@ Reference call surface and its counterpart here:
     apply_unary!(sys, op!)             apply_unary(sys, op, **params)
     sum(p -> energy(p), sys.particles) energy(sys, **params)
 
+``Batch`` (include/sph_batch.h, no counterpart in the reference) holds many
+independent systems, periodic ones included, and steps them together.
+
 Operators cannot be arbitrary closures across the C ABI; the built-in ones
 are registered by the reference name they replace (``OPERATORS``).
 Everything runs through the HIP library; there is no CPU fallback.
@@ -30,7 +33,7 @@ __all__ = [
     "step", "OPERATORS", "SPHError", "find_rho", "find_rho0", "internal_force",
     "internal_force_rho0", "balance_of_mass", "reset_rho", "reset_a", "find_pressure",
     "find_pressure_rho0", "find_pressure_continuity", "accelerate", "move",
-    "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL",
+    "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
 ]
 
 FLUID = 0.0
@@ -359,3 +362,160 @@ def init_reference_density(sys, **params):
     create_cell_list(sys)
     sys.set("rho0", 0.0)
     apply(sys, find_rho0, self=True, **params)
+
+
+_SCHEMES = {"wcsph_se": N.SPH_SCHEME_WCSPH_SE, "wcsph_se_rho0": N.SPH_SCHEME_WCSPH_SE_RHO0}
+
+
+class Batch:
+    """Many independent particle systems stepped together (include/sph_batch.h).
+
+    Each system keeps its own cell geometry, parameters and reference
+    particle order and evolves exactly as it would alone; systems join with
+    ``add`` and leave with ``retire`` between calls.  Every system of a batch
+    has the same dimension.
+    """
+
+    def __init__(self, precision="mixed"):
+        mode = {"fp64": N.SPH_FP64, "mixed": N.SPH_MIXED}[precision]
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        check(lib().sph_batch_create(ctypes.byref(self._h), mode))
+
+    def close(self):
+        if self._h:
+            lib().sph_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, x, h, bbox_lo, bbox_hi, periodic=False, v=None, rho=None, P=None,
+            type=None, **params):
+        """Add a system; returns its id.  ``params`` are those of
+        make_params (m, rho0, c, alpha, hs, mu, nu, g, dt)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+
+        def opt(a, shape):
+            if a is None:
+                return None
+            return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), shape))
+
+        v, rho, P, t = opt(v, (n, 3)), opt(rho, (n,)), opt(P, (n,)), opt(type, (n,))
+        d = N.sph_system_desc()
+        d.h = float(h)
+        for i in range(3):
+            d.bbox_min[i] = float(bbox_lo[i])
+            d.bbox_max[i] = float(bbox_hi[i])
+        d.periodic = 1 if periodic else 0
+        d.params = make_params(**params)
+        sid = ctypes.c_int()
+        check(lib().sph_batch_add(self._h, ctypes.byref(d), n, _as_ptr(x),
+                                  _as_ptr(v) if v is not None else None,
+                                  _as_ptr(rho) if rho is not None else None,
+                                  _as_ptr(P) if P is not None else None,
+                                  _as_ptr(t) if t is not None else None, ctypes.byref(sid)))
+        return sid.value
+
+    def add_state(self, st, periodic=False):
+        """Add a workload state of ``states`` (dam break, Taylor-Green)."""
+        return self.add(st.x, st.R, st.bbox_lo, st.bbox_hi, periodic=periodic, v=st.v,
+                        rho=st.rho, P=st.P, type=st.type, **st.params())
+
+    def retire(self, sid):
+        check(lib().sph_batch_retire(self._h, int(sid)))
+
+    def size(self):
+        """(live systems, their particles)."""
+        ns = ctypes.c_int()
+        n = ctypes.c_int64()
+        check(lib().sph_batch_size(self._h, ctypes.byref(ns), ctypes.byref(n)))
+        return ns.value, n.value
+
+    def count(self, sid):
+        n = ctypes.c_int64()
+        check(lib().sph_batch_count(self._h, int(sid), ctypes.byref(n)))
+        return n.value
+
+    def get(self, sid, name):
+        n = self.count(sid)
+        out = np.empty((n, 3) if name in N.VECTOR_FIELDS else (n,), dtype=np.float64)
+        check(lib().sph_batch_get_field(self._h, int(sid), N.FIELDS[name], _as_ptr(out), n))
+        return out
+
+    def set(self, sid, name, values):
+        n = self.count(sid)
+        shape = (n, 3) if name in N.VECTOR_FIELDS else (n,)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), shape))
+        check(lib().sph_batch_set_field(self._h, int(sid), N.FIELDS[name], _as_ptr(a), n))
+
+    def reduce(self, sid, what):
+        out = (ctypes.c_double * 3)()
+        check(lib().sph_batch_reduce(self._h, int(sid), what, out))
+        return list(out)
+
+    def energy(self, sid, kind="wcsph_rho0"):
+        what = {"wcsph": N.SPH_REDUCE_ENERGY, "collision": N.SPH_REDUCE_ENERGY_COLLISION,
+                "wcsph_rho0": N.SPH_REDUCE_ENERGY_RHO0}[kind]
+        return self.reduce(sid, what)[0]
+
+    def momentum(self, sid):
+        return np.array(self.reduce(sid, N.SPH_REDUCE_MOMENTUM))
+
+    def build_cells(self):
+        check(lib().sph_batch_build_cells(self._h))
+
+    def neighbour_pairs(self, sid):
+        """Accepted ordered pairs of one system under the current cell lists
+        (reference indices of that system), unspecified order."""
+        n = ctypes.c_int64()
+        check(lib().sph_batch_neighbour_pairs(self._h, int(sid), None, 0, ctypes.byref(n)))
+        total = n.value
+        out = np.empty((max(total, 1), 2), dtype=np.int64)
+        check(lib().sph_batch_neighbour_pairs(
+            self._h, int(sid), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), total,
+            ctypes.byref(n)))
+        return out[:total]
+
+    def init_rho0(self):
+        """find_rho0! (self = true) for the systems added since the last call."""
+        check(lib().sph_batch_init_rho0(self._h))
+
+    def step(self, nsteps, scheme="wcsph_se_rho0"):
+        """nsteps fused steps of every system; returns the density-pass pairs."""
+        pairs = ctypes.c_int64()
+        check(lib().sph_batch_step(self._h, _SCHEMES[scheme], int(nsteps), ctypes.byref(pairs)))
+        return pairs.value
+
+    def step_pairs(self):
+        d = ctypes.c_int64()
+        f = ctypes.c_int64()
+        check(lib().sph_step_pairs(self._h, ctypes.byref(d), ctypes.byref(f)))
+        return d.value, f.value
+
+    def set_tile(self, T):
+        check(lib().sph_batch_set_tile(self._h, int(T)))
+
+    def autotune(self):
+        best = ctypes.c_int()
+        check(lib().sph_batch_autotune(self._h, ctypes.byref(best)))
+        return best.value
+
+    def footprint(self):
+        """Device bytes held by the batch."""
+        b = ctypes.c_int64()
+        check(lib().sph_batch_footprint(self._h, ctypes.byref(b)))
+        return b.value
+
+    def profile(self, enable=True):
+        check(lib().sph_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib().sph_profile_read(self._h, kernel, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

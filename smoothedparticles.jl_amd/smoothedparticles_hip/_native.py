@@ -1,4 +1,5 @@
-"""ctypes declarations of libsph_hip (include/sph.h, include/sph_setup.h).
+"""ctypes declarations of libsph_hip (include/sph.h, include/sph_batch.h,
+include/sph_setup.h).
 
 The library is built in-tree (``make -C smoothedparticles.jl_amd/csrc``) and
 loaded from this directory.  There is no fallback: if the library is missing
@@ -69,6 +70,17 @@ class sph_params(ctypes.Structure):
     ]
 
 
+class sph_system_desc(ctypes.Structure):
+    _fields_ = [
+        ("h", ctypes.c_double),
+        ("bbox_min", ctypes.c_double * 3),
+        ("bbox_max", ctypes.c_double * 3),
+        ("periodic", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("params", sph_params),
+    ]
+
+
 _d = ctypes.c_double
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -100,6 +112,23 @@ SIGNATURES = {
     "sph_profile": (_i, [_p, _i]),
     "sph_profile_read": (_i, [_p, _i, _pd, _pi64]),
     "sph_last_error": (ctypes.c_char_p, []),
+    "sph_batch_create": (_i, [ctypes.POINTER(_p), _i]),
+    "sph_batch_destroy": (_i, [_p]),
+    "sph_batch_add": (_i, [_p, ctypes.POINTER(sph_system_desc), _i64, _pd, _pd, _pd, _pd, _pd,
+                           ctypes.POINTER(_i)]),
+    "sph_batch_retire": (_i, [_p, _i]),
+    "sph_batch_size": (_i, [_p, ctypes.POINTER(_i), _pi64]),
+    "sph_batch_count": (_i, [_p, _i, _pi64]),
+    "sph_batch_get_field": (_i, [_p, _i, _i, _pd, _i64]),
+    "sph_batch_set_field": (_i, [_p, _i, _i, _pd, _i64]),
+    "sph_batch_reduce": (_i, [_p, _i, _i, _pd]),
+    "sph_batch_build_cells": (_i, [_p]),
+    "sph_batch_neighbour_pairs": (_i, [_p, _i, _pi64, _i64, _pi64]),
+    "sph_batch_init_rho0": (_i, [_p]),
+    "sph_batch_step": (_i, [_p, _i, _i, _pi64]),
+    "sph_batch_set_tile": (_i, [_p, _i]),
+    "sph_batch_autotune": (_i, [_p, ctypes.POINTER(_i)]),
+    "sph_batch_footprint": (_i, [_p, _pi64]),
     "sph_lattice_box": (_i64, [_i, _d, _pd, _pd, _pd, _i64]),
     "sph_lattice_wall_layer": (_i64, [_i, _d, _pd, _pd, _d, _i, _d, _pd, _i64]),
     "sph_jitter": (None, [_pd, _i64, _i, _d, ctypes.c_uint64]),

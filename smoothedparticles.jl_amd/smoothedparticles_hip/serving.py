@@ -1,0 +1,237 @@
+"""Continuous-batching request loop over Batch handles (not in the reference).
+
+Requests are independent simulations (2D/3D dam break, periodic
+Taylor-Green) that arrive over time.  The server keeps one device batch per
+dimension; a request is admitted as soon as it has arrived and its
+estimated device footprint fits the memory budget, its state stays on the
+device while it runs, and it leaves the batch when its step count is done.
+Requests that arrive while others run join the running batch between two
+stepping calls, so their work shares every launch with the others.
+
+Each batch is advanced in quanta of at most ``quantum`` steps, cut short so
+that no request runs past its step count; a request's trajectory therefore
+does not depend on what else is in the batch (the batch gives the same
+bits as a batch of one with the same tile size).
+"""
+import math
+import time
+
+import numpy as np
+
+from . import Batch, states
+
+KINDS = ("dam_break_2d", "dam_break_3d", "taylor_green")
+
+# Device bytes per particle used for admission: particle columns, two
+# buffers, build scratch and pair masks, with the capacity doubling of the
+# particle arrays (measured footprints stay below these).
+BYTES_PER_PARTICLE = {2: 1200, 3: 2400}
+
+
+class Request:
+    """One simulation request."""
+
+    def __init__(self, rid, kind, n_target, steps, arrival, re=None):
+        self.rid = rid
+        self.kind = kind
+        self.n_target = n_target
+        self.steps = steps
+        self.arrival = arrival
+        self.re = re
+        self.state = None
+        self.sid = None
+        self.done_steps = 0
+        self.admitted = None
+        self.finished = None
+        self.result = None
+
+    @property
+    def dim(self):
+        return 3 if self.kind == "dam_break_3d" else 2
+
+    @property
+    def periodic(self):
+        return self.kind == "taylor_green"
+
+    def build(self):
+        """The initial state (built on the host when first needed)."""
+        if self.state is None:
+            self.state = make_state(self.kind, self.n_target, self.re)
+        return self.state
+
+    @property
+    def n(self):
+        return len(self.build().x)
+
+    def footprint(self):
+        return BYTES_PER_PARTICLE[self.dim] * self.n
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def make_state(kind, n_target, re=None):
+    """A workload state of about n_target particles."""
+    if kind == "taylor_green":
+        side = max(16, int(round(math.sqrt(n_target))))
+        return states.taylor_green(side, re if re is not None else 100.0)
+    if kind == "dam_break_2d":
+        # fluid lattice (1/dr + 1)(2/dr + 1) ~ 2/dr^2; the walls come on top
+        return states.dam_break(2, math.sqrt(2.0 / n_target))
+    if kind == "dam_break_3d":
+        return states.dam_break(3, (2.0 / n_target) ** (1.0 / 3.0))
+    raise ValueError(kind)
+
+
+def make_requests(count, rate, seed=0, n_lo=1000, n_hi=500000, steps_lo=200, steps_hi=2000,
+                  kinds=KINDS):
+    """A seeded request stream: Poisson arrivals of the given rate (per
+    second), particle counts log-uniform in [n_lo, n_hi], step counts uniform
+    in [steps_lo, steps_hi], kinds uniform, Taylor-Green Re uniform in
+    [50, 1000]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
+        kind = kinds[int(rng.integers(len(kinds)))]
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        re = float(rng.uniform(50.0, 1000.0))
+        out.append(Request(i, kind, n, steps, t, re))
+    return out
+
+
+def summary(st, b, sid):
+    """Result of a finished request: kinetic energy and momentum."""
+    v = b.get(sid, "v")
+    return {"n": int(len(v)), "kinetic": 0.5 * st.m * float(np.sum(v * v)),
+            "momentum": (st.m * v.sum(axis=0)).tolist()}
+
+
+class Server:
+    """Continuous batching of requests on one GPU.
+
+    precision   "mixed" or "fp64"
+    budget      device bytes the admitted requests may use (estimate)
+    quantum     steps per stepping call of a batch
+    tiles       targets per tile for each dimension (fixed, so a request's
+                results do not depend on the batch it ran in)
+    """
+
+    def __init__(self, precision="mixed", budget=16 << 30, quantum=50, tiles=None):
+        self.precision = precision
+        self.budget = budget
+        self.quantum = quantum
+        self.tiles = tiles or {2: 32, 3: 64}
+        self.batches = {}
+        self.active = {2: [], 3: []}
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.peak_active = 0
+        self.pairs = 0
+        self.particle_steps = 0
+
+    def _batch(self, dim):
+        b = self.batches.get(dim)
+        if b is None:
+            b = Batch(self.precision)
+            b.set_tile(self.tiles[dim])
+            self.batches[dim] = b
+        return b
+
+    def fits(self, r):
+        return self.in_use + r.footprint() <= self.budget
+
+    def admit(self, r, now):
+        st = r.build()
+        b = self._batch(r.dim)
+        r.sid = b.add_state(st, periodic=r.periodic)
+        r.admitted = now
+        self.active[r.dim].append(r)
+        self.in_use += r.footprint()
+        self.peak_in_use = max(self.peak_in_use, self.in_use)
+
+    def step_all(self, clock):
+        """Advance every non-empty batch by one quantum; returns the requests
+        that finished."""
+        done = []
+        for dim in (2, 3):
+            act = self.active[dim]
+            if not act:
+                continue
+            b = self.batches[dim]
+            b.init_rho0()  # the reference density of the requests that just joined
+            k = min(self.quantum, min(r.steps - r.done_steps for r in act))
+            b.step(k)
+            dp, fp = b.step_pairs()
+            self.pairs += dp + fp
+            self.particle_steps += k * sum(r.n for r in act)
+            now = clock()
+            keep = []
+            for r in act:
+                r.done_steps += k
+                if r.done_steps >= r.steps:
+                    r.result = summary(r.state, b, r.sid)
+                    r.finished = now
+                    b.retire(r.sid)
+                    self.in_use -= r.footprint()
+                    r.state = None  # host arrays are no longer needed
+                    done.append(r)
+                else:
+                    keep.append(r)
+            self.active[dim] = keep
+        return done
+
+    def run(self, requests, realtime=True, on_done=None):
+        """Serve a request stream.  With realtime=True arrivals follow the
+        wall clock (the server sleeps while idle); otherwise every request
+        is available at once in arrival order.  Returns the requests with
+        admission and completion times (seconds since the start)."""
+        queue = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+
+        def clock():
+            return time.perf_counter() - t0
+
+        qi = 0
+        finished = []
+        while qi < len(queue) or any(self.active.values()):
+            now = clock()
+            # FIFO admission: arrived and fits the budget (an oversize
+            # request runs alone)
+            while qi < len(queue) and (not realtime or queue[qi].arrival <= now):
+                r = queue[qi]
+                idle = not any(self.active.values())
+                if not (self.fits(r) or idle):
+                    break
+                if not realtime:
+                    r.arrival = now
+                self.admit(r, now)
+                qi += 1
+            nact = sum(len(a) for a in self.active.values())
+            self.peak_active = max(self.peak_active, nact)
+            if nact == 0:
+                if qi < len(queue):
+                    time.sleep(max(0.0, min(0.05, queue[qi].arrival - clock())))
+                continue
+            for r in self.step_all(clock):
+                finished.append(r)
+                if on_done:
+                    on_done(r)
+        self.wall = clock()
+        return finished
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches = {}
+
+
+def latency_stats(finished):
+    lat = np.array([r.latency for r in finished], dtype=np.float64)
+    if len(lat) == 0:
+        return {"p50_s": None, "p99_s": None, "mean_s": None}
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean())}

@@ -6,6 +6,9 @@ cubic lattice: a fluid block in a tank, wall particles from
 BoundaryLayer(tank, grid, 3.5 dr) without the lid, hydrostatic P and rho
 for every particle (collapse_dry.jl:97-100), and a seeded uniform jitter
 of +-1e-6 dr on the fluid positions.
+
+Taylor-Green vortex (not in the reference): a periodic unit box, square
+lattice, u = sin(2 pi x) cos(2 pi y), v = -cos(2 pi x) sin(2 pi y).
 """
 import ctypes
 import math
@@ -135,3 +138,50 @@ def collision_2d(dr=2.0e-2, c=20.0, v0=1.0):
                  type=np.zeros(len(x)), bbox_lo=np.array([-10.0, -10.0, 0.0]),
                  bbox_hi=np.array([10.0, 10.0, 0.0]), R=R, hs=R, m=1000.0 * dr * dr,
                  rho0=1000.0, c=c, alpha=0.0, g=np.zeros(3), dt=0.1 * R / c)
+
+
+def taylor_green(n, Re, R_over_dr=4.8, c=10.0, rho0=1.0):
+    """2D Taylor-Green vortex on an n x n lattice of the periodic unit box.
+
+    dr = 1/n, particles at ((i + 1/2) dr, (j + 1/2) dr); U = L = 1, so the
+    kinematic viscosity is nu = 1/Re, applied through the laminar term of the
+    internal force (mu = rho0 nu).  c = 10 U (Mach 0.1), hs = 2.4 dr, neighbour
+    radius R = R_over_dr dr, dt = 0.1 hs / c, m = rho0 dr^2, no gravity, no
+    artificial viscosity.  The kinetic energy of the exact solution decays
+    as KE(t) = KE(0) exp(-16 pi^2 nu t).
+    """
+    dr = 1.0 / n
+    g = (np.arange(n, dtype=np.float64) + 0.5) * dr
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    x = np.zeros((n * n, 3))
+    x[:, 0] = X.ravel()
+    x[:, 1] = Y.ravel()
+    v = np.zeros_like(x)
+    tp = 2.0 * math.pi
+    v[:, 0] = np.sin(tp * x[:, 0]) * np.cos(tp * x[:, 1])
+    v[:, 1] = -np.cos(tp * x[:, 0]) * np.sin(tp * x[:, 1])
+    hs = 2.4 * dr
+    nu = 1.0 / Re
+    st = State(dim=2, dr=dr, x=x, v=v, rho=np.full(n * n, rho0), P=np.zeros(n * n),
+               type=np.zeros(n * n), n_fluid=n * n, n_wall=0,
+               bbox_lo=np.zeros(3), bbox_hi=np.array([1.0, 1.0, 0.0]), R=R_over_dr * dr,
+               hs=hs, m=rho0 * dr * dr, rho0=rho0, c=c, alpha=0.0, g=np.zeros(3),
+               dt=0.1 * hs / c, Re=Re, nu=nu, mu=rho0 * nu)
+    st.params = lambda: dict(m=st.m, rho0=rho0, c=c, alpha=0.0, hs=hs, mu=st.mu, nu=0.0,
+                             g=(0.0, 0.0, 0.0), dt=st.dt, coeff=1.0)
+    return st
+
+
+def taylor_green_kinetic(st, v):
+    """Kinetic energy sum m |v|^2 / 2 of a Taylor-Green state."""
+    return 0.5 * st.m * float(np.sum(v * v))
+
+
+def taylor_green_batch(count=256, n_lo=2000, n_hi=60000, re_lo=50.0, re_hi=1000.0, seed=0):
+    """The periodic batch workload: particle counts N_i drawn log-uniform in
+    [n_lo, n_hi] (lattice side round(sqrt(N_i))) and Re_i uniform in
+    [re_lo, re_hi], from a seeded PCG64 stream.  Returns [(side, Re), ...]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    N = np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), size=count))
+    Re = rng.uniform(re_lo, re_hi, size=count)
+    return [(int(round(math.sqrt(float(a)))), float(b)) for a, b in zip(N, Re)]

@@ -72,7 +72,7 @@ def test_dambreak_2d_counts():
 
 def header_functions():
     names = []
-    for h in ("sph.h", "sph_setup.h"):
+    for h in ("sph.h", "sph_batch.h", "sph_setup.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         names += re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(sph_[a-z_0-9]+)\s*\(", txt, re.M)
     return sorted(set(names))
@@ -93,6 +93,12 @@ def test_params_struct_layout():
     assert p.m == 2.0 and p.g[1] == -9.8 and p.coeff == 1.0
     with pytest.raises(TypeError):
         sp.make_params(bogus=1.0)
+
+
+def test_system_desc_layout():
+    # double h; double bbox_min[3], bbox_max[3]; int periodic, pad; sph_params
+    assert ctypes.sizeof(N.sph_system_desc) == 8 + 48 + 8 + 12 * 8
+    assert N.sph_system_desc.params.offset == 64
 
 
 def test_operator_registry():
