@@ -13,7 +13,21 @@ The same workload in plain FP64 (the drop-in path) is timed beside it, and
 the FP64 CPU oracle (a C port of the reference algorithm, OpenMP over
 particles) is timed on one step of the same workload on the host cores.
 
+Beside the headline the same JSON line carries:
+  accuracy_vs_fp64  mixed against FP64 on the device after 100 and 1000 steps
+                    (relative density, velocity and total-energy error);
+  tg_batch          256 periodic 2D Taylor-Green boxes (N log-uniform in
+                    [2k, 60k], Re uniform in [50, 1000]) advanced 500 steps as
+                    one batch, mixed and FP64, with the kinetic-energy decay
+                    against the FP64 run and the closed form;
+  serving           a seeded Poisson stream of 2D/3D dam-break and
+                    Taylor-Green requests (1k-500k particles, 200-2000 steps)
+                    served by continuous batching in the mixed mode: p50/p99
+                    completion latency and pair-interactions/s, beside the same
+                    requests run one at a time in plain FP64.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
+       [--no-fp64] [--no-cpu] [--no-accuracy] [--no-batch] [--no-serve]
 """
 import argparse
 import json
@@ -29,12 +43,21 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
-# Algorithmic work of one force pass (per particle / per candidate / per
-# accepted pair): bytes read and written once in sorted order, 9 flops per
-# candidate test, about 45 flops per accepted force pair.
-FORCE_BYTES_PER_PARTICLE = {"mixed": 48.0, "fp64": 92.0}
+# Algorithmic work of the two pair passes of a step.
+#   density pass: every candidate of every target is tested once (3 sub,
+#     1 mul, 2 fma, 1 compare, 2 band compares = 9 flops), every accepted
+#     pair adds m W (about 15 flops); bytes: positions read and rho written
+#     once in sorted order (the accepted-candidate masks it writes are
+#     scratch, not algorithmic).
+#   force pass: only the accepted pairs of fluid targets (the masks of the
+#     density pass say which), about 45 flops each (pressure gradient, Monaghan
+#     viscosity, compensated sums); bytes: x, v, rho, P/rho^2, type read and a
+#     written once.
 FLOPS_PER_CANDIDATE = 9.0
+FLOPS_PER_DENSITY_PAIR = 15.0
 FLOPS_PER_FORCE_PAIR = 45.0
+DENSITY_BYTES_PER_PARTICLE = {"mixed": 32.0, "fp64": 32.0}
+FORCE_BYTES_PER_PARTICLE = {"mixed": 92.0, "fp64": 92.0}
 
 # Density summation with the per-particle reference density rho0_p set by
 # one find_rho0! pass at t = 0 (tests/test_collision_2d.jl:123-126): the EOS
@@ -84,6 +107,7 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
         elapsed=el, pairs=pairs, value=2.0 * pairs / el, n=n, tile=T,
         force_ms=force_ms / max(force_n, 1), density_ms=dens_ms / max(dens_n, 1),
         build_ms=build_ms / max(build_n, 1), force_pairs_per_pass=force_pairs / steps,
+        density_pairs_per_pass=dens_pairs / steps,
         cand_fluid=cand_fluid, cand_all=cand_all,
     )
 
@@ -123,6 +147,123 @@ def accuracy(st, checkpoints=(100, 1000)):
     return out
 
 
+def tg_batch(steps=500, count=256, seed=0):
+    """The batched periodic Taylor-Green workload, mixed and FP64."""
+    import math
+
+    import numpy as np
+
+    import smoothedparticles_hip as sp
+    from smoothedparticles_hip import states
+
+    spec = states.taylor_green_batch(count, seed=seed)
+    sts = [states.taylor_green(n, Re) for n, Re in spec]
+    # warm the kernels up on a throwaway batch of a few boxes
+    wb = sp.Batch("mixed")
+    for st in sts[:4]:
+        wb.add_state(st, periodic=True)
+    wb.init_rho0()
+    wb.step(2)
+    wb.close()
+    res = {}
+    ke = {}
+    for prec in ("mixed", "fp64"):
+        b = sp.Batch(prec)
+        ids = [b.add_state(st, periodic=True) for st in sts]
+        b.init_rho0()
+        T = b.autotune()
+        t0 = time.perf_counter()
+        b.step(steps)
+        el = time.perf_counter() - t0
+        dp, fp = b.step_pairs()
+        ke[prec] = np.array([states.taylor_green_kinetic(st, b.get(i, "v"))
+                             for st, i in zip(sts, ids)])
+        res[prec] = dict(value=(dp + fp) / el, ms_per_step=1e3 * el / steps, tile_targets=T,
+                         footprint_GB=b.footprint() / 1e9)
+        b.close()
+        log(f"tg_batch {prec}: {res[prec]}")
+    ke0 = np.array([states.taylor_green_kinetic(st, st.v) for st in sts])
+    exact = np.array([math.exp(-16 * math.pi ** 2 * st.nu * steps * st.dt) for st in sts])
+    rel_fp64 = np.abs(ke["mixed"] / ke["fp64"] - 1)
+    rel_exact = np.abs(ke["fp64"] / ke0 / exact - 1)
+    return {
+        "workload": f"{count} periodic 2D Taylor-Green unit boxes, N log-uniform [2k, 60k] "
+                    f"(total {sum(len(st.x) for st in sts)}), Re uniform [50, 1000], R=4.8dr, "
+                    f"{steps} steps, one launch per stage for all boxes",
+        "value": res["mixed"]["value"], "unit": "pair-interactions/s",
+        "ms_per_step": res["mixed"]["ms_per_step"], "tile_targets": res["mixed"]["tile_targets"],
+        "fp64_value": res["fp64"]["value"], "fp64_ms_per_step": res["fp64"]["ms_per_step"],
+        "mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+        "ke_rel_vs_fp64_max": float(rel_fp64.max()),
+        "ke_rel_vs_closed_form_median": float(np.median(rel_exact)),
+        "ke_rel_vs_closed_form_max": float(rel_exact.max()),
+    }
+
+
+def serve(count=48, rate=2.5, seed=0, budget_gb=32.0):
+    """The served request stream, mixed with continuous batching, and the
+    same requests one at a time in plain FP64 (single-system handles)."""
+    import numpy as np
+
+    import smoothedparticles_hip as sp
+    from smoothedparticles_hip import serving
+
+    reqs = serving.make_requests(count, rate, seed=seed)
+    t0 = time.perf_counter()
+    for r in reqs:
+        r.build()
+    log(f"serving: {count} request states built in {time.perf_counter() - t0:.1f} s")
+    total_particles = sum(r.n for r in reqs)
+    srv = serving.Server("mixed", budget=int(budget_gb * (1 << 30)), quantum=50)
+    fin = srv.run(reqs, realtime=True)
+    lat = serving.latency_stats(fin)
+    out = {
+        "workload": f"{count} requests, Poisson rate {rate}/s, kinds {list(serving.KINDS)}, "
+                    f"1k-500k particles (total {total_particles}), 200-2000 steps, seed {seed}",
+        "value": srv.pairs / srv.wall, "unit": "pair-interactions/s",
+        "wall_s": srv.wall, "p50_s": lat["p50_s"], "p99_s": lat["p99_s"],
+        "mean_latency_s": lat["mean_s"], "peak_concurrent": srv.peak_active,
+        "budget_GB": budget_gb, "peak_admitted_GB": srv.peak_in_use / 1e9,
+    }
+    srv.close()
+    log(f"serving mixed: {out}")
+    # sequential plain FP64: service time of each request alone, then the
+    # same arrivals through a FIFO single server
+    service, pairs_tot = [], 0
+    for r in sorted(reqs, key=lambda r: r.rid):
+        st = r.build()
+        p = st.params()
+        t1 = time.perf_counter()
+        if r.periodic:
+            g = sp.Batch("fp64")
+            sid = g.add_state(st, periodic=True)
+            g.init_rho0()
+            g.step(r.steps)
+            dp, fp = g.step_pairs()
+        else:
+            g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, "fp64")
+            g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
+            sp.init_reference_density(g, **p)
+            sp.step(g, r.steps, scheme=SCHEME, **p)
+            dp, fp = g.step_pairs()
+        g.close()
+        service.append(time.perf_counter() - t1)
+        pairs_tot += dp + fp
+        r.state = None
+    arrivals = [r.arrival for r in sorted(reqs, key=lambda r: r.rid)]
+    finish, lat_seq = 0.0, []
+    for a, sv in zip(arrivals, service):
+        finish = max(finish, a) + sv
+        lat_seq.append(finish - a)
+    out["fp64_sequential_value"] = pairs_tot / sum(service)
+    out["fp64_sequential_p50_s"] = float(np.percentile(lat_seq, 50))
+    out["fp64_sequential_p99_s"] = float(np.percentile(lat_seq, 99))
+    out["speedup_vs_fp64_sequential"] = out["value"] / out["fp64_sequential_value"]
+    log(f"serving fp64 sequential: {out['fp64_sequential_value']:.4g} pair-int/s, "
+        f"p50 {out['fp64_sequential_p50_s']:.2f} s, p99 {out['fp64_sequential_p99_s']:.2f} s")
+    return out
+
+
 def cpu_baseline(st):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -150,8 +291,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--accuracy", action="store_true",
-                    help="also report mixed-vs-FP64 errors at steps 100 and 1000")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the mixed-vs-FP64 errors at steps 100 and 1000")
+    ap.add_argument("--no-batch", action="store_true", help="skip the Taylor-Green batch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -175,14 +318,41 @@ def main():
 
     prec = args.precision
     peak = PEAK_FP32_TFLOPS if prec == "mixed" else PEAK_FP64_TFLOPS
-    flops = FLOPS_PER_CANDIDATE * res["cand_fluid"] + FLOPS_PER_FORCE_PAIR * res["force_pairs_per_pass"]
-    achieved = flops / (res["force_ms"] * 1e-3) / 1e12
-    hbm_bytes = FORCE_BYTES_PER_PARTICLE[prec] * res["n"]
-    hbm_gbs = hbm_bytes / (res["force_ms"] * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"force_pass_{args.workload}_{prec}_traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    def roofline(kernel):
+        if kernel == "density":
+            flops = (FLOPS_PER_CANDIDATE * res["cand_all"]
+                     + FLOPS_PER_DENSITY_PAIR * res["density_pairs_per_pass"])
+            ms, bpp = res["density_ms"], DENSITY_BYTES_PER_PARTICLE[prec]
+            name = "density pass (candidate test, pair masks, rho = sum m W)"
+        else:
+            flops = FLOPS_PER_FORCE_PAIR * res["force_pairs_per_pass"]
+            ms, bpp = res["force_ms"], FORCE_BYTES_PER_PARTICLE[prec]
+            name = "force pass (pressure gradient + artificial viscosity over masked pairs)"
+        achieved = flops / (ms * 1e-3) / 1e12
+        hbm_gbs = bpp * res["n"] / (ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"{kernel}_pass_{args.workload}_{prec}_traffic.json")
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        return {
+            "bound": "mfma",
+            "ceiling": "vector ALU: the pair kernels issue no MFMA; peak is the published "
+                       f"{'FP32' if prec == 'mixed' else 'FP64'} rate",
+            "kernel": name,
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "flops_per_launch": flops,
+            "hbm_algorithmic_GBps": hbm_gbs,
+            "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
+            "avg_launch_ms": ms,
+        }
+
+    dominant = "density" if res["density_ms"] >= res["force_ms"] else "force"
+    secondary = "force" if dominant == "density" else "density"
 
     out = {
         "metric": "pair-interactions/s, FP32-mixed vs FP64 plain at stated error; p50/p99 latency",
@@ -206,27 +376,19 @@ def main():
             "tile_targets": res["tile"],
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "mfma",
-            "ceiling": "vector ALU: the force kernel issues no MFMA; peak is the published "
-                       f"{'FP32' if prec == 'mixed' else 'FP64'} rate",
-            "kernel": "force pass (pressure gradient + artificial viscosity)",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": achieved / peak,
-            "traffic": traffic,
-            "hbm_algorithmic_GBps": hbm_gbs,
-            "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
-            "avg_launch_ms": res["force_ms"],
-        },
+        "roofline": roofline(dominant),
+        "roofline_" + secondary: roofline(secondary),
     }
     if other is not None:
         out["fp64_value"] = other["value"]
         out["mixed_over_fp64"] = res["value"] / other["value"]
         out["fp64_ms_per_step"] = 1e3 * other["elapsed"] / args.steps
-    if args.accuracy:
+    if not args.no_accuracy:
         out["accuracy_vs_fp64"] = accuracy(st)
+    if not args.no_batch:
+        out["tg_batch"] = tg_batch()
+    if not args.no_serve:
+        out["serving"] = serve()
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(st)

@@ -397,7 +397,7 @@ PairCtx make_ctx(sph_system* s)
     c.tiles = s->tiles;
     c.cell_start = s->cell_start;
     c.meta = s->meta;
-    c.queue = s->meta + META_QUEUE;
+    c.queue = s->queue;
     c.cols = s->cols[s->cur];
     c.refidx = s->icols[s->cur] + I_REF * s->cap;
     c.prho2 = s->prho2;
@@ -452,7 +452,7 @@ int run_pair(sph_system* s, int op, int self, int overwrite, unsigned long long*
     hipEvent_t ev[2];
     prof_begin(s, ev);
     reset_queue(s);
-    launch_pair(op, s->precision, s->rows, s->flat, c, pair_grid(ntiles), s->stream);
+    launch_pair(op, s->precision, s->rows, s->flat, s->periodic, c, pair_grid(ntiles), s->stream);
     prof_end(s, prof_kernel, ev);
     HIPC(hipGetLastError());
     return 0;
@@ -500,6 +500,7 @@ int new_handle(int precision, sph_system** out)
         rc = fail(SPH_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
     if (!rc && !(rc = dalloc(s, &s->meta, META_SLOTS)) && !(rc = dalloc(s, &s->pair_counter, 4)) &&
+        !(rc = dalloc(s, &s->queue, NQUEUE * QSTRIDE)) &&
         !(rc = dalloc(s, &s->red_out, 4)) && !(rc = dalloc(s, &s->mask_total, 1))) {
         e = hipHostMalloc((void**)&s->meta_host, sizeof(int) * META_SLOTS, hipHostMallocDefault);
         if (e != hipSuccess) {
@@ -533,6 +534,7 @@ int destroy(sph_system* s)
     free_particle_arrays(s);
     free_cell_arrays(s);
     dfree(s, s->meta, META_SLOTS); dfree(s, s->pair_counter, 4); dfree(s, s->red_out, 4);
+    dfree(s, s->queue, NQUEUE * QSTRIDE);
     dfree(s, s->masks, s->mask_cap); dfree(s, s->mask_total, 1);
     dfree(s, s->sysd, s->sysd_cap); dfree(s, s->sys_cnt, SYSCNT * s->sys_cnt_cap);
     if (s->meta_host) (void)hipHostFree(s->meta_host);
@@ -583,6 +585,7 @@ int add_system(sph_system* s, const Geo& g, const Phys& ph, int* slot_out)
     }
     s->rows = rows;
     s->flat = slot == 0 ? g.flat : (s->flat && g.flat);
+    s->periodic = slot == 0 ? g.periodic : (s->periodic || g.periodic);
     enqueue_fill_int(s, s->cell_sys + d.cell_base, slot, g.key_max);
     enqueue_set_int(s, s->sys_cnt + SYSCNT * slot + SC_N, 0);
     HIPC(hipGetLastError());
@@ -765,7 +768,12 @@ int compact(sph_system* s)
     s->id_of_slot = nid;
     s->rho0_pending = npend;
     s->cells_total = cells;
-    if (nsys(s) == 0) s->flat = 1;
+    s->flat = 1;
+    s->periodic = 0;
+    for (const SysDesc& d : s->sys_host) {
+        s->flat = s->flat && d.g.flat;
+        s->periodic = s->periodic || d.g.periodic;
+    }
     if ((rc = upload_sysd(s))) return rc;
     for (int i = 0; i < nsys(s); ++i) {
         int* c = s->sys_cnt_host + SYSCNT * i;
@@ -862,7 +870,7 @@ int neighbour_pairs(sph_system* s, int only_slot, int64_t* pairs, int64_t cap, i
             e = hipMemsetAsync(s->pair_counter, 0, sizeof(unsigned long long), s->stream);
             if (e == hipSuccess) {
                 reset_queue(s);
-                launch_pair(P_EMIT, s->precision, s->rows, s->flat, c,
+                launch_pair(P_EMIT, s->precision, s->rows, s->flat, s->periodic, c,
                             pair_grid(s->meta_host[META_NTILES]), s->stream);
                 e = hipGetLastError();
             }
@@ -949,8 +957,11 @@ int autotune(sph_system* s, int* best)
 }  // namespace
 
 namespace sph {
-__global__ void k_reset_queue(int* meta) { meta[META_QUEUE] = 0; }
-void reset_queue(sph_system* s) { k_reset_queue<<<1, 1, 0, s->stream>>>(s->meta); }
+__global__ void k_reset_queue(int* queue)
+{
+    if (threadIdx.x < NQUEUE) queue[threadIdx.x * QSTRIDE] = 0;
+}
+void reset_queue(sph_system* s) { k_reset_queue<<<1, 64, 0, s->stream>>>(s->queue); }
 }  // namespace sph
 
 extern "C" {

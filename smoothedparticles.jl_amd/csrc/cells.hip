@@ -355,8 +355,8 @@ __global__ void k_remove_seq(const int* meta, int nsys, int cells_total, const i
     }
 }
 
-// 64-candidate chunks one target of global cell c walks (the same loop as
-// the pair kernels: per stencil row and range, ceil(length / 64)).
+// 64-candidate chunks one target of global cell c walks: the pair kernels
+// concatenate the cell's stencil ranges and cut the list into chunks of 64.
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__ sysd,
                                                      const int* __restrict__ cell_sys,
@@ -372,10 +372,10 @@ __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__
         for (int row = 0; row < ROWS; ++row) {
             Seg seg[2];
             const int ns = row_segments<ROWS>(d, cell_start, lc, row, seg);
-            for (int q = 0; q < ns; ++q) n += (seg[q].j1 - seg[q].j0 + 63) / 64;
+            for (int q = 0; q < ns; ++q) n += seg[q].j1 - seg[q].j0;
         }
     }
-    chunks[c] = n;
+    chunks[c] = (n + 63) / 64;
 }
 
 // Tile table: tile k of cell c covers targets cell_start[c] + k*T ...; its

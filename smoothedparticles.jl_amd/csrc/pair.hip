@@ -1,42 +1,58 @@
 // Pair-interaction kernels: the device form of _apply_binary! /
 // apply_binary! (src/core.jl:94-129) for the registered operators.
 //
-// Work decomposition.  A tile is up to T targets of one cell (T = 8..64).
-// One 64-lane wave handles a tile: lane = (target t, slice s), t = lane % T,
-// s = lane / T, with S = 64 / T slices.  The candidates of a target are the
-// cells key + key_diff of the reference; with di innermost, each of the 3
-// (2D) or 9 (3D) (dj, dk) rows is the contiguous cell range
-// [key + L1*dj + L1*L2*dk - 1, ... + 1], clamped to [0, key_max) exactly as
-// the reference's linear bounds test (src/core.jl:98), so rows wrap and
-// small grids visit aliased cells as often as the reference does.  Since
-// the particles are cell-sorted, each row is one contiguous particle range.
-// The wave streams it through LDS in chunks of 64 (one coalesced load per
-// lane).  Each chunk is handled in two phases:
-//   1. mask: every lane tests the chunk entries jj = s (mod S) and sets bit
-//      jj of a 64-bit mask for each accepted candidate -- or, after a
-//      density pass on the same cell list, loads that pass's mask;
+// Work decomposition.  A tile is up to T targets of one cell (T = 8..64, a
+// template parameter).  One 64-lane wave handles a tile: lane = (target t,
+// slice s), t = lane % T, s = lane / T, with S = 64 / T slices.
+//
+// Candidates.  The candidates of a target are the cells key + key_diff of
+// the reference; with di innermost, each of the 3 (2D) or 9 (3D) (dj, dk)
+// rows is the contiguous cell range [key + L1*dj + L1*L2*dk - 1, ... + 1],
+// clamped to [0, key_max) exactly as the reference's linear bounds test
+// (src/core.jl:98), so rows wrap and small grids visit aliased cells as
+// often as the reference does.  Since the particles are cell-sorted, each
+// row is one contiguous particle range (periodic systems: at most two, with
+// an image shift each).  All targets of a tile share the cell, hence the
+// ranges: the tile's candidate list is the concatenation of its ranges,
+// streamed through LDS in chunks of 64 that may span several ranges.
+//
+// Each chunk is handled in two phases:
+//   1. mask: lane (t, s) tests the chunk slots s + S*j (j < T) against
+//      target t and sets bit j of its T-bit mask for slot s + S*j -- or,
+//      after a density pass on the same cell list, loads that pass's mask.
+//      Interleaving the slices keeps the accepted counts of the lanes
+//      alike (neighbouring slots are neighbouring particles);
 //   2. interact: every lane walks the set bits of its mask, so the pair
 //      arithmetic runs once per accepted pair, not once per candidate.
 // At the end the S slices of a target are combined with wave shuffles.
+// The masks of target t for a chunk form one 64-bit word (the T-bit mask
+// of slice s in bits [s*T, s*T + T)), stored at
+// mask_base(tile) + chunk * T + t.
 //
 // Predicate (src/core.jl:104-105, src/algebra.jl:49-60): accept q != p when
 // RN(sqrt((d1*d1 + d2*d2) + d3*d3)) <= h.  sqrt is correctly rounded and
 // monotonic, so this equals s <= s_thr with s_thr = max{s : RN(sqrt(s)) <= h}
 // (computed on the host).  The FP64 kernels evaluate s without FMA
 // (-ffp-contract=off), i.e. bit-exactly as the reference.  The mixed
-// kernels evaluate s in FP32 on tile-relative coordinates and fall back to
-// the exact FP64 test inside a guard band around s_thr, so their neighbour
-// sets (and masks) are the reference's too.
+// kernels test |q|^2 - 2 p.q <= s_thr (1 + BAND) - |p|^2 in FP32 on
+// tile-relative coordinates (three FMAs per candidate), and every candidate
+// accepted there whose direct FP32 squared distance lies above
+// s_thr (1 - BAND) is decided by the exact FP64 test, so their neighbour
+// sets (and masks) are the reference's too.  The identity exclusion p != q
+// is applied once per chunk to the slots that hold p.
+#include <type_traits>
+
 #include "sph_internal.h"
 
 namespace sph {
 
 #define COL(c, f, i) ((c).cols[(long long)(f) * (c).cap + (i)])
 
-// Guard band of the mixed predicate, relative to s_thr.  The FP32 squared
-// distance of tile-relative coordinates (|coords| <= 2*sqrt(3)*h) is within
-// ~2e-6 * h^2 of the exact one; the band is 5x wider.
-static constexpr double BAND = 1.0e-5;
+// Guard band of the mixed predicate, relative to s_thr.  Tile-relative
+// coordinates are below 4 h in magnitude; the FP32 expanded form
+// |q|^2 - 2 p.q + |p|^2 is then within ~1e-5 * h^2 of the exact squared
+// distance and the direct form within ~1e-6 * h^2; the band is 10x wider.
+static constexpr double BAND = 1.0e-4;
 
 __device__ inline void count_pairs(const PairCtx& c, unsigned n)
 {
@@ -55,24 +71,177 @@ __device__ inline void two_sum_add(float& s, float& e, float x)
     s = t;
 }
 
-// Bits jj with jj % S == 0 (S = 1, 2, 4, 8).
-__device__ inline unsigned long long slice_pattern(int S)
+// Tile queues.  The tile range [0, ntiles) is cut into NQUEUE contiguous
+// parts, one per queue head.  A wave starts on queue blockIdx % NQUEUE (the
+// XCD it runs on, so neighbouring tiles share that XCD's L2), takes QGRAB
+// tiles per atomic, and when its queue is empty moves on to the next ones.
+// Every wave leaves once all queues are past their ends.  The state is
+// wave-uniform (readfirstlane), so the tile record, the system id and the
+// system's parameters are scalar loads held in SGPRs.
+struct TileCursor {
+    int q0, k, cur, end;
+};
+
+__device__ inline void tile_cursor_init(TileCursor& tc)
 {
-    return ~0ull / ((1ull << S) - 1ull);
+    tc.q0 = (int)(blockIdx.x % NQUEUE);
+    tc.k = 0;
+    tc.cur = tc.end = 0;
 }
 
-// Tiles come from a work queue (one atomic per tile); every wave leaves
-// the loop once the queue passes the tile count.
-__device__ inline int next_tile(const PairCtx& c)
+__device__ inline int next_tile(const PairCtx& c, int ntiles, TileCursor& tc)
 {
-    int t = 0;
-    if (__lane_id() == 0) t = atomicAdd(c.queue, 1);
-    return __shfl(t, 0);
+    if (tc.cur < tc.end) return tc.cur++;
+    while (tc.k < NQUEUE) {
+        const int q = (tc.q0 + tc.k) % NQUEUE;
+        const int lo = (int)((long long)ntiles * q / NQUEUE);
+        const int hi = (int)((long long)ntiles * (q + 1) / NQUEUE);
+        int start = 0;
+        if (__lane_id() == 0) start = atomicAdd(&c.queue[q * QSTRIDE], QGRAB);
+        start = __builtin_amdgcn_readfirstlane(start);
+        if (start < hi - lo) {
+            tc.cur = lo + start;
+            tc.end = min(hi, lo + start + QGRAB);
+            return tc.cur++;
+        }
+        ++tc.k;
+    }
+    return ntiles;
 }
 
 __device__ inline long long mask_base(const int4& td)
 {
     return (long long)(unsigned)td.z | ((long long)td.w << 32);
+}
+
+// Per-lane mask of T slots.
+template <int T>
+using MaskT = typename std::conditional<(T > 32), unsigned long long, unsigned>::type;
+
+template <int T>
+__device__ inline int lowest_bit(MaskT<T> m)
+{
+    if constexpr (T > 32) return __ffsll((long long)m) - 1;
+    else return __ffs(m) - 1;
+}
+
+template <int T>
+__device__ inline int bit_count(MaskT<T> m)
+{
+    if constexpr (T > 32) return __popcll(m);
+    else return __popc(m);
+}
+
+// ------------------------------------------------------------ candidates --
+// The candidate ranges of a tile: its cell's stencil rows, concatenated.
+// pref[i] candidates precede range i; a chunk's slot l holds candidate
+// base + l, whose range i satisfies pref[i] <= base + l < pref[i + 1].
+template <int ROWS>
+struct Cands {
+    static constexpr int MAXR = 2 * ROWS;
+    int j0[MAXR];
+    int pref[MAXR + 1];
+    double sx[MAXR], sy[MAXR], sz[MAXR];  // image shift (periodic systems)
+    int q[64];                            // particle index of each staged slot
+    int rg[64];                           // range of each staged slot
+};
+
+// Uniform work (every lane computes the same values); lane 0 writes.
+// Returns the number of ranges; *total receives the candidate count.
+template <int ROWS, bool PERIODIC>
+__device__ inline int build_cands(const SysDesc& d, const int* cell_start, int lc, Cands<ROWS>& L,
+                                  int& total)
+{
+    const bool w = __lane_id() == 0;
+    int n = 0, tot = 0;
+    for (int row = 0; row < ROWS; ++row) {
+        Seg segs[2];
+        const int ns = row_segments<ROWS, PERIODIC>(d, cell_start, lc, row, segs);
+        for (int k = 0; k < 2; ++k) {
+            if (k < ns) {
+                if (w) {
+                    L.j0[n] = segs[k].j0;
+                    L.pref[n] = tot;
+                    if (PERIODIC) {
+                        L.sx[n] = segs[k].sx;
+                        L.sy[n] = segs[k].sy;
+                        L.sz[n] = segs[k].sz;
+                    }
+                }
+                tot += segs[k].j1 - segs[k].j0;
+                ++n;
+            }
+        }
+    }
+    if (w) L.pref[n] = tot;
+    total = tot;
+    return n;
+}
+
+// Slot of chunk `base` that holds candidate g (g < total): its range and
+// particle index.  Ranges are scanned from `cur`, the range of slot 0.
+template <int ROWS>
+__device__ inline int slot_range(const Cands<ROWS>& L, int cur, int g)
+{
+    int i = cur;
+    while (L.pref[i + 1] <= g) ++i;
+    return i;
+}
+
+// Clear the bits of the slots of this chunk that hold the target itself
+// (the reference skips p === q, src/core.jl:105).  A small grid may list
+// the target's cell in several rows; every occurrence is cleared.
+template <int ROWS, int T>
+__device__ inline void exclude_self(const Cands<ROWS>& L, int nr, int cur, int base, int p, int s,
+                                    MaskT<T>& m)
+{
+    for (int i = cur; i < nr && L.pref[i] < base + 64; ++i) {
+        const int off = p - L.j0[i];
+        if (off >= 0 && off < L.pref[i + 1] - L.pref[i]) {
+            constexpr int S = 64 / T;
+            const int slot = L.pref[i] + off - base;
+            if (slot >= 0 && slot < 64 && slot % S == s) m &= ~(MaskT<T>(1) << (slot / S));
+        }
+    }
+}
+
+// Range of the next chunk's slot 0.
+template <int ROWS>
+__device__ inline int advance_range(const Cands<ROWS>& L, int nr, int cur, int next_base)
+{
+    while (cur < nr && L.pref[cur + 1] <= next_base) ++cur;
+    return cur;
+}
+
+template <int ROWS>
+__device__ inline Seg range_shift(const Cands<ROWS>& L, int i)
+{
+    Seg g;
+    g.j0 = g.j1 = 0;
+    g.sx = L.sx[i];
+    g.sy = L.sy[i];
+    g.sz = L.sz[i];
+    return g;
+}
+
+// Load the stored mask word of target t for a chunk; slice s keeps its bits.
+template <int T>
+__device__ inline MaskT<T> load_mask(const PairCtx& c, long long mbase, int chunk, int t, int s)
+{
+    const unsigned long long word = c.masks[mbase + (long long)chunk * T + t];
+    if constexpr (T == 64) return word;
+    else return (MaskT<T>)((word >> (s * T)) & ((1ull << T) - 1ull));
+}
+
+// Assemble the 64-bit word of target t from its S slices; slice 0 stores it.
+template <int T>
+__device__ inline void store_mask(const PairCtx& c, long long mbase, int chunk, int t, int s,
+                                  MaskT<T> m)
+{
+    unsigned long long full = (unsigned long long)m << (T == 64 ? 0 : s * T);
+#pragma unroll
+    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
+    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
 }
 
 // ================================================================ FP64 ====
@@ -84,11 +253,19 @@ struct Lds64 {
 };
 
 // Candidate positions are staged with the periodic image shift of their
-// range added (zero for reference systems, where x + 0.0 == x).
+// range added.
+template <bool P>
 __device__ inline void stage_pos64(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
 {
-    s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
-    s.zr[l].x = COL(c, C_Z, q) + g.sz;
+    s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
+    s.zr[l].x = shifted<P>(COL(c, C_Z, q), g.sz);
+}
+
+// Slots past the end of the list: infinitely far, rejected by the test.
+__device__ inline void pad_pos64(int l, Lds64& s)
+{
+    s.xy[l] = make_double2(INFINITY, INFINITY);
+    s.zr[l].x = INFINITY;
 }
 
 // rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0).
@@ -106,9 +283,10 @@ struct Density64 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        stage_pos64(c, q, l, g, s);
+        stage_pos64<P>(c, q, l, g, s);
     }
     static __device__ void init(Acc& a) { a.w = 0.0; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds64&,
@@ -155,10 +333,11 @@ struct Force64 {
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
+    template <bool P>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
-        s.zr[l] = make_double2(COL(c, C_Z, q) + g.sz, COL(c, C_RHO, q));
+        s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
+        s.zr[l] = make_double2(shifted<P>(COL(c, C_Z, q), g.sz), COL(c, C_RHO, q));
         s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
         s.v2a[l] = make_double2(COL(c, C_VZ, q), c.prho2[q]);
     }
@@ -219,9 +398,10 @@ struct ForceRho064 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p); t.P = COL(c, C_P, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        stage_pos64(c, q, l, g, s);
+        stage_pos64<P>(c, q, l, g, s);
         s.v2a[l].y = COL(c, C_P, q);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
@@ -268,10 +448,11 @@ struct Continuity64 {
         t.rho = COL(c, C_RHO, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        s.xy[l] = make_double2(COL(c, C_X, q) + g.sx, COL(c, C_Y, q) + g.sy);
-        s.zr[l] = make_double2(COL(c, C_Z, q) + g.sz, COL(c, C_RHO, q));
+        s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
+        s.zr[l] = make_double2(shifted<P>(COL(c, C_Z, q), g.sz), COL(c, C_RHO, q));
         s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
         s.v2a[l].x = COL(c, C_VZ, q);
     }
@@ -312,9 +493,10 @@ struct Pairs64 {
         t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
     {
-        stage_pos64(c, q, l, g, s);
+        stage_pos64<P>(c, q, l, g, s);
     }
     static __device__ void init(Acc&) {}
     static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds64&,
@@ -330,19 +512,18 @@ struct Pairs64 {
                                   bool) {}
 };
 
-template <int ROWS, bool FLAT, class Op>
+template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
-                              Lds64& lds)
+                              Lds64& lds, Cands<ROWS>& L)
 {
-    const int4 td = c.tiles[tile];
+    const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
-    const SysDesc& d = c.sys[c.cell_sys[cell]];
+    const SysDesc d = ld_const(c.sys + ld_const(c.cell_sys + cell));
     if (c.only_flagged && !d.pass_on) return;
     const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
-    const int cend = c.cell_start[cell + 1];
+    const int cend = ld_const(c.cell_start + cell + 1);
     const int lane = threadIdx.x;
-    const int T = c.T, S = 64 / T;
     const int t = lane & (T - 1), s = lane / T;
     const int p = first + t;
     const bool valid = p < cend;
@@ -350,77 +531,83 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     Op::load(c, valid ? p : first, tg);
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
-    const unsigned long long sbits = slice_pattern(S) << s;
     const double s_thr = d.g.s_thr;
+    int total;
+    const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
     unsigned npairs = 0;
-    int chunk = 0;
-    for (int row = 0; row < ROWS; ++row) {
-        Seg segs[2];
-        const int nseg = row_segments<ROWS>(d, c.cell_start, lc, row, segs);
-        for (int sg = 0; sg < nseg; ++sg) {
-            const Seg seg = segs[sg];
-            for (int b = seg.j0; b < seg.j1; b += 64, ++chunk) {
-                const int nb = min(64, seg.j1 - b);
-                if (lane < nb) Op::stage(c, b + lane, lane, seg, lds);
-                __syncthreads();
-                unsigned long long m = 0ull;
-                if (load_masks) {
-                    if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
-                } else {
-                    for (int j = 0; j < nb; j += S) {
-                        const int jj = j + s;
-                        if (jj < nb) {
-                            const double2 xy = lds.xy[jj];
-                            const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                            double ss = dx * dx + dy * dy;
-                            if (!FLAT) {
-                                const double dz = tg.z - lds.zr[jj].x;
-                                ss = ss + dz * dz;
-                            }
-                            if (act && ss <= s_thr && b + jj != p) m |= 1ull << jj;
-                        }
-                    }
-                    if (store_masks) {
-                        unsigned long long full = m;
-                        for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
-                        if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
-                    }
+    int cur = 0, chunk = 0;
+    for (int base = 0; base < total; base += 64, ++chunk) {
+        const int g = base + lane;
+        if (g < total) {
+            const int i = slot_range(L, cur, g);
+            const int q = L.j0[i] + (g - L.pref[i]);
+            Seg sh{};
+            if (PERIODIC) sh = range_shift(L, i);
+            Op::template stage<PERIODIC>(c, q, lane, sh, lds);
+            L.q[lane] = q;
+        } else {
+            pad_pos64(lane, lds);
+        }
+        __syncthreads();
+        MaskT<T> m = 0;
+        if (load_masks) {
+            if (act) m = load_mask<T>(c, mbase, chunk, t, s);
+        } else {
+#pragma unroll 4
+            for (int j = 0; j < T; ++j) {
+                const int jj = s + (64 / T) * j;
+                const double2 xy = lds.xy[jj];
+                const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                double ss = dx * dx + dy * dy;
+                if (!FLAT) {
+                    const double dz = tg.z - lds.zr[jj].x;
+                    ss = ss + dz * dz;
                 }
-                npairs += (unsigned)__popcll(m);
-                if (Op::INTERACT) {
-                    while (__any(m != 0ull)) {
-                        if (m) {
-                            const int jj = __ffsll((long long)m) - 1;
-                            m &= m - 1ull;
-                            const double2 xy = lds.xy[jj];
-                            const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                            const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
-                            double ss = dx * dx + dy * dy;
-                            if (!FLAT) ss = ss + dz * dz;
-                            Op::interact(c, d, tg, lds, jj, p, b + jj, dx, dy, dz, ss, acc);
-                        }
-                    }
+                if (ss <= s_thr) m |= MaskT<T>(1) << j;
+            }
+            if (!act) m = 0;
+            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
+            if (store_masks) store_mask<T>(c, mbase, chunk, t, s, m);
+        }
+        npairs += (unsigned)bit_count<T>(m);
+        if (Op::INTERACT) {
+            while (__any(m != 0)) {
+                if (m) {
+                    const int j = lowest_bit<T>(m);
+                    m &= m - 1;
+                    const int jj = s + (64 / T) * j;
+                    const double2 xy = lds.xy[jj];
+                    const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                    const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
+                    double ss = dx * dx + dy * dy;
+                    if (!FLAT) ss = ss + dz * dz;
+                    Op::interact(c, d, tg, lds, jj, p, L.q[jj], dx, dy, dz, ss, acc);
                 }
-                __syncthreads();
             }
         }
+        cur = advance_range(L, nr, cur, base + 64);
+        __syncthreads();
     }
     if (c.pair_counter) count_pairs(c, npairs);
     Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
 
-template <int ROWS, bool FLAT, class Op>
+template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 {
     __shared__ Lds64 lds;
-    const int ntiles = c.meta[META_NTILES];
-    const bool ok = c.masks != nullptr && c.meta[META_MASK_OK] != 0;
+    __shared__ Cands<ROWS> L;
+    const int ntiles = ld_const(c.meta + META_NTILES);
+    const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
-    for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
-        tile64<ROWS, FLAT, Op>(c, tile, load_masks, store_masks, lds);
+    TileCursor tc;
+    tile_cursor_init(tc);
+    for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
+        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L);
 }
 
 // =============================================================== mixed ====
@@ -429,13 +616,33 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 // partial sums are plain FP32 within a 64-candidate chunk and folded into a
 // compensated (sum, error) pair after every chunk.
 
-// LDS staging (FP32): (x, y, z, P/rho^2), (vx, vy, vz, rho).
+// LDS staging (FP32): (x, y, z, |x|^2) -- (x, y, z, P/rho^2) for the
+// force pass -- and, for the force pass, (vx, vy, vz, rho).
 struct Lds32 {
-    float4 a[64], v[64];
+    float4 pos[64], v[64];
 };
+
+// Relative position of candidate q (with its image shift) and |.|^2.
+template <bool P, bool FLAT>
+__device__ inline float4 rel_pos32(const PairCtx& c, int q, const Seg& g, double ox, double oy,
+                                   double oz)
+{
+    const float x = (float)(shifted<P>(COL(c, C_X, q), g.sx) - ox);
+    const float y = (float)(shifted<P>(COL(c, C_Y, q), g.sy) - oy);
+    const float z = FLAT ? 0.0f : (float)(shifted<P>(COL(c, C_Z, q), g.sz) - oz);
+    return make_float4(x, y, z, fmaf(z, z, fmaf(y, y, x * x)));
+}
+
+// Slots past the end of the list fail every test: |x|^2 = inf, and the
+// coordinates are far away for operators that recompute |x|^2.
+__device__ inline void pad_pos32(int l, Lds32& s)
+{
+    s.pos[l] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, INFINITY);
+}
 
 struct Density32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool STAGE_Q2 = true;  // pos.w = |x|^2
     struct Tgt {
         float x, y, z;
     };
@@ -452,12 +659,11 @@ struct Density32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P, bool FLAT>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
                                  double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
-                             (float)((COL(c, C_Y, q) + g.sy) - oy),
-                             (float)((COL(c, C_Z, q) + g.sz) - oz), 0.0f);
+        s.pos[l] = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
     }
     static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
@@ -486,6 +692,7 @@ struct Density32 {
 
 struct Force32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
     struct Tgt {
         float x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -507,12 +714,13 @@ struct Force32 {
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
+    template <bool P, bool FLAT>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
                                  double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
-                             (float)((COL(c, C_Y, q) + g.sy) - oy),
-                             (float)((COL(c, C_Z, q) + g.sz) - oz), (float)c.prho2[q]);
+        float4 a = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
+        a.w = (float)c.prho2[q];
+        s.pos[l] = a;
         s.v[l] = make_float4((float)COL(c, C_VX, q), (float)COL(c, C_VY, q),
                              (float)COL(c, C_VZ, q), (float)COL(c, C_RHO, q));
     }
@@ -529,8 +737,8 @@ struct Force32 {
         const float r = __builtin_sqrtf(ss);
         const float u = 1.0f - r * (float)d.g.inv_h;
         const float u3 = u * u * u;
-        const float4 a = s.a[jj], v = s.v[jj];
-        float A = t.pr + a.w;
+        const float4 v = s.v[jj];
+        float A = t.pr + s.pos[jj].w;
         const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
         if (d.ph.avk != 0.0) {
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
@@ -581,6 +789,7 @@ struct Force32 {
 template <bool EMIT>
 struct Pairs32 {
     static constexpr bool INTERACT = EMIT;
+    static constexpr bool STAGE_Q2 = true;
     struct Tgt {
         float x, y, z;
     };
@@ -597,12 +806,11 @@ struct Pairs32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
+    template <bool P, bool FLAT>
     static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
                                  double oy, double oz, Lds32& s)
     {
-        s.a[l] = make_float4((float)((COL(c, C_X, q) + g.sx) - ox),
-                             (float)((COL(c, C_Y, q) + g.sy) - oy),
-                             (float)((COL(c, C_Z, q) + g.sz) - oz), 0.0f);
+        s.pos[l] = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
     }
     static __device__ void init(Acc&) {}
     static __device__ void zero(Part&) {}
@@ -620,119 +828,129 @@ struct Pairs32 {
                                   int, bool, int) {}
 };
 
-template <int ROWS, bool FLAT, class Op>
+template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, Lds32& lds)
+                              bool store_masks, Lds32& lds, Cands<ROWS>& L)
 {
-    const int4 td = c.tiles[tile];
+    const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
-    const SysDesc& d = c.sys[c.cell_sys[cell]];
+    const SysDesc d = ld_const(c.sys + ld_const(c.cell_sys + cell));
     if (c.only_flagged && !d.pass_on) return;
     const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
-    const int cend = c.cell_start[cell + 1];
+    const int cend = ld_const(c.cell_start + cell + 1);
     const int lane = threadIdx.x;
-    const int T = c.T, S = 64 / T;
     const int t = lane & (T - 1), s = lane / T;
     const int p = first + t;
     const bool valid = p < cend;
     const int pp = valid ? p : first;
-    const double ox = COL(c, C_X, first), oy = COL(c, C_Y, first), oz = COL(c, C_Z, first);
+    // tile origin (positions are not written by the pair kernels)
+    const double ox = ld_const(&COL(c, C_X, first)), oy = ld_const(&COL(c, C_Y, first)),
+                 oz = ld_const(&COL(c, C_Z, first));
     typename Op::Tgt tg;
     Op::load(c, pp, ox, oy, oz, tg);
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
-    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
     const double s_thr = d.g.s_thr;
     const float thr_lo = (float)(s_thr * (1.0 - BAND));
-    const float thr_hi = (float)(s_thr * (1.0 + BAND));
-    const unsigned long long sbits = slice_pattern(S) << s;
+    // expanded test: |q|^2 - 2 p.q <= s_thr (1 + BAND) - |p|^2
+    const float tz = FLAT ? 0.0f : tg.z;
+    const float thr_p = (float)(s_thr * (1.0 + BAND)) - fmaf(tz, tz, fmaf(tg.y, tg.y, tg.x * tg.x));
+    const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
+    int total;
+    const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
     unsigned npairs = 0;
-    int chunk = 0;
-    for (int row = 0; row < ROWS; ++row) {
-        Seg segs[2];
-        const int nseg = row_segments<ROWS>(d, c.cell_start, lc, row, segs);
-        for (int sg = 0; sg < nseg; ++sg) {
-            const Seg seg = segs[sg];
-            for (int b = seg.j0; b < seg.j1; b += 64, ++chunk) {
-                const int nb = min(64, seg.j1 - b);
-                if (lane < nb) Op::stage(c, b + lane, lane, seg, ox, oy, oz, lds);
-                __syncthreads();
-                unsigned long long m = 0ull;
-                if (load_masks) {
-                    if (act) m = c.masks[mbase + (long long)chunk * T + t] & sbits;
-                } else {
-                    for (int j = 0; j < nb; j += S) {
-                        const int jj = j + s;
-                        const bool in = jj < nb;
-                        const int jr = in ? jj : 0;
-                        const float4 a = lds.a[jr];
-                        const float dx = tg.x - a.x, dy = tg.y - a.y;
-                        const float dz = FLAT ? 0.0f : tg.z - a.z;
-                        const float ss =
-                            FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        const int q = b + jr;
-                        bool ok = ss <= thr_lo;
-                        const bool band = in && act && ss > thr_lo && ss <= thr_hi;
-                        if (__any(band)) {
-                            if (band) {
-                                // exact FP64 predicate of the reference
-                                const double e1 = px - (COL(c, C_X, q) + seg.sx);
-                                const double e2 = py - (COL(c, C_Y, q) + seg.sy);
-                                double es = e1 * e1 + e2 * e2;
-                                if (!FLAT) {
-                                    const double e3 = pz - (COL(c, C_Z, q) + seg.sz);
-                                    es = es + e3 * e3;
-                                }
-                                ok = es <= s_thr;
-                            }
-                        }
-                        if (in && act && ok && q != p) m |= 1ull << jj;
+    int cur = 0, chunk = 0;
+    for (int base = 0; base < total; base += 64, ++chunk) {
+        const int g = base + lane;
+        if (g < total) {
+            const int i = slot_range(L, cur, g);
+            const int q = L.j0[i] + (g - L.pref[i]);
+            Seg sh{};
+            if (PERIODIC) sh = range_shift(L, i);
+            Op::template stage<PERIODIC, FLAT>(c, q, lane, sh, ox, oy, oz, lds);
+            L.q[lane] = q;
+            if (PERIODIC) L.rg[lane] = i;
+        } else {
+            pad_pos32(lane, lds);
+        }
+        __syncthreads();
+        MaskT<T> m = 0;
+        if (load_masks) {
+            if (act) m = load_mask<T>(c, mbase, chunk, t, s);
+        } else {
+#pragma unroll 4
+            for (int j = 0; j < T; ++j) {
+                const float4 a = lds.pos[s + (64 / T) * j];
+                const float q2 = Op::STAGE_Q2 ? a.w : fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x));
+                float e = fmaf(mx, a.x, q2);
+                e = fmaf(my, a.y, e);
+                if (!FLAT) e = fmaf(mz, a.z, e);
+                if (e <= thr_p) m |= MaskT<T>(1) << j;
+            }
+            if (!act) m = 0;
+            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
+        }
+        // accepted candidates: the guard band is decided in FP64, then the
+        // pair arithmetic
+        MaskT<T> keep = m;
+        typename Op::Part part;
+        Op::zero(part);
+        while (__any(m != 0)) {
+            if (m) {
+                const int j = lowest_bit<T>(m);
+                m &= m - 1;
+                const int jj = s + (64 / T) * j;
+                const float4 a = lds.pos[jj];
+                const float dx = tg.x - a.x, dy = tg.y - a.y;
+                const float dz = FLAT ? 0.0f : tg.z - a.z;
+                const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                bool ok = true;
+                if (!load_masks && ss > thr_lo) {
+                    // exact FP64 predicate of the reference
+                    const int q = L.q[jj];
+                    Seg sh{};
+                    if (PERIODIC) sh = range_shift(L, L.rg[jj]);
+                    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp);
+                    const double e1 = px - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
+                    const double e2 = py - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
+                    double es = e1 * e1 + e2 * e2;
+                    if (!FLAT) {
+                        const double e3 = COL(c, C_Z, pp) - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
+                        es = es + e3 * e3;
                     }
-                    if (store_masks) {
-                        unsigned long long full = m;
-                        for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
-                        if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
-                    }
+                    ok = es <= s_thr;
+                    if (!ok) keep &= ~(MaskT<T>(1) << j);
                 }
-                npairs += (unsigned)__popcll(m);
-                if (Op::INTERACT) {
-                    typename Op::Part part;
-                    Op::zero(part);
-                    while (__any(m != 0ull)) {
-                        if (m) {
-                            const int jj = __ffsll((long long)m) - 1;
-                            m &= m - 1ull;
-                            const float4 a = lds.a[jj];
-                            const float dx = tg.x - a.x, dy = tg.y - a.y;
-                            const float dz = FLAT ? 0.0f : tg.z - a.z;
-                            const float ss =
-                                FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                            Op::interact(c, d, tg, lds, jj, p, b + jj, dx, dy, dz, ss, part);
-                        }
-                    }
-                    Op::fold(acc, part);
-                }
-                __syncthreads();
+                if (Op::INTERACT && ok) Op::interact(c, d, tg, lds, jj, p, L.q[jj], dx, dy, dz, ss, part);
             }
         }
+        Op::fold(acc, part);
+        npairs += (unsigned)bit_count<T>(keep);
+        if (store_masks) store_mask<T>(c, mbase, chunk, t, s, keep);
+        cur = advance_range(L, nr, cur, base + 64);
+        __syncthreads();
     }
     if (c.pair_counter) count_pairs(c, npairs);
     Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
-template <int ROWS, bool FLAT, class Op>
+template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
 {
     __shared__ Lds32 lds;
-    const int ntiles = c.meta[META_NTILES];
-    const bool ok = c.masks != nullptr && c.meta[META_MASK_OK] != 0;
+    __shared__ Cands<ROWS> L;
+    const int ntiles = ld_const(c.meta + META_NTILES);
+    const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
-    for (int tile = next_tile(c); tile < ntiles; tile = next_tile(c))
-        tile32<ROWS, FLAT, Op>(c, tile, out_col, load_masks, store_masks, lds);
+    TileCursor tc;
+    tile_cursor_init(tc);
+    for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
+        tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L);
 }
 
 // ========================================================= diagnostics ====
@@ -778,42 +996,59 @@ void launch_candidates(int rows, const PairCtx& c, const int* key, unsigned long
 
 // ============================================================ dispatch ====
 
-template <int ROWS, bool FLAT>
-static void launch_rows(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
+template <int ROWS, bool FLAT, bool PER, int T>
+static void launch_tile(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
 {
     if (precision == SPH_FP64) {
         switch (op) {
-        case P_DENSITY: k_pair64<ROWS, FLAT, Density64<C_RHO>><<<grid, 64, 0, st>>>(c); break;
-        case P_DENSITY_RHO0: k_pair64<ROWS, FLAT, Density64<C_RHO0>><<<grid, 64, 0, st>>>(c); break;
-        case P_FORCE: k_pair64<ROWS, FLAT, Force64><<<grid, 64, 0, st>>>(c); break;
-        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, ForceRho064><<<grid, 64, 0, st>>>(c); break;
-        case P_CONTINUITY: k_pair64<ROWS, FLAT, Continuity64><<<grid, 64, 0, st>>>(c); break;
-        case P_COUNT: k_pair64<ROWS, FLAT, Pairs64<false>><<<grid, 64, 0, st>>>(c); break;
-        case P_EMIT: k_pair64<ROWS, FLAT, Pairs64<true>><<<grid, 64, 0, st>>>(c); break;
+        case P_DENSITY: k_pair64<ROWS, FLAT, PER, T, Density64<C_RHO>><<<grid, 64, 0, st>>>(c); break;
+        case P_DENSITY_RHO0: k_pair64<ROWS, FLAT, PER, T, Density64<C_RHO0>><<<grid, 64, 0, st>>>(c); break;
+        case P_FORCE: k_pair64<ROWS, FLAT, PER, T, Force64><<<grid, 64, 0, st>>>(c); break;
+        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, PER, T, ForceRho064><<<grid, 64, 0, st>>>(c); break;
+        case P_CONTINUITY: k_pair64<ROWS, FLAT, PER, T, Continuity64><<<grid, 64, 0, st>>>(c); break;
+        case P_COUNT: k_pair64<ROWS, FLAT, PER, T, Pairs64<false>><<<grid, 64, 0, st>>>(c); break;
+        case P_EMIT: k_pair64<ROWS, FLAT, PER, T, Pairs64<true>><<<grid, 64, 0, st>>>(c); break;
         }
     } else {
         switch (op) {
-        case P_DENSITY: k_pair32<ROWS, FLAT, Density32><<<grid, 64, 0, st>>>(c, C_RHO); break;
-        case P_DENSITY_RHO0: k_pair32<ROWS, FLAT, Density32><<<grid, 64, 0, st>>>(c, C_RHO0); break;
-        case P_FORCE: k_pair32<ROWS, FLAT, Force32><<<grid, 64, 0, st>>>(c, 0); break;
-        case P_COUNT: k_pair32<ROWS, FLAT, Pairs32<false>><<<grid, 64, 0, st>>>(c, 0); break;
-        case P_EMIT: k_pair32<ROWS, FLAT, Pairs32<true>><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_DENSITY: k_pair32<ROWS, FLAT, PER, T, Density32><<<grid, 64, 0, st>>>(c, C_RHO); break;
+        case P_DENSITY_RHO0: k_pair32<ROWS, FLAT, PER, T, Density32><<<grid, 64, 0, st>>>(c, C_RHO0); break;
+        case P_FORCE: k_pair32<ROWS, FLAT, PER, T, Force32><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_COUNT: k_pair32<ROWS, FLAT, PER, T, Pairs32<false>><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_EMIT: k_pair32<ROWS, FLAT, PER, T, Pairs32<true>><<<grid, 64, 0, st>>>(c, 0); break;
         // operators without a mixed form run in FP64
-        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, ForceRho064><<<grid, 64, 0, st>>>(c); break;
-        case P_CONTINUITY: k_pair64<ROWS, FLAT, Continuity64><<<grid, 64, 0, st>>>(c); break;
+        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, PER, T, ForceRho064><<<grid, 64, 0, st>>>(c); break;
+        case P_CONTINUITY: k_pair64<ROWS, FLAT, PER, T, Continuity64><<<grid, 64, 0, st>>>(c); break;
         }
     }
 }
 
-void launch_pair(int op, int precision, int rows, int flat, const PairCtx& c, int grid,
-                 hipStream_t st)
+template <int ROWS, bool FLAT, bool PER>
+static void launch_rows(int op, int precision, const PairCtx& c, int grid, hipStream_t st)
+{
+    switch (c.T) {
+    case 8: launch_tile<ROWS, FLAT, PER, 8>(op, precision, c, grid, st); break;
+    case 16: launch_tile<ROWS, FLAT, PER, 16>(op, precision, c, grid, st); break;
+    case 32: launch_tile<ROWS, FLAT, PER, 32>(op, precision, c, grid, st); break;
+    default: launch_tile<ROWS, FLAT, PER, 64>(op, precision, c, grid, st); break;
+    }
+}
+
+void launch_pair(int op, int precision, int rows, int flat, int periodic, const PairCtx& c,
+                 int grid, hipStream_t st)
 {
     if (grid <= 0) return;
     if (rows == 3) {
-        if (flat) launch_rows<3, true>(op, precision, c, grid, st);
-        else launch_rows<3, false>(op, precision, c, grid, st);
+        if (flat) {
+            if (periodic) launch_rows<3, true, true>(op, precision, c, grid, st);
+            else launch_rows<3, true, false>(op, precision, c, grid, st);
+        } else {
+            if (periodic) launch_rows<3, false, true>(op, precision, c, grid, st);
+            else launch_rows<3, false, false>(op, precision, c, grid, st);
+        }
     } else {
-        launch_rows<9, false>(op, precision, c, grid, st);
+        if (periodic) launch_rows<9, false, true>(op, precision, c, grid, st);
+        else launch_rows<9, false, false>(op, precision, c, grid, st);
     }
 }
 

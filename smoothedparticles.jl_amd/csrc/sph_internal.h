@@ -41,7 +41,7 @@ enum Meta {
     META_NREM = 2,    // particles removed by the last build
     META_NTILES = 3,  // tiles of the pair kernels
     META_NCELLS = 4,  // non-empty cells
-    META_QUEUE = 5,   // work-queue head of the pair kernels
+    META_QUEUE = 5,   // (unused; the pair kernels' tile queues are separate)
     META_MASK_OK = 6, // the pair-mask buffer holds this cell list's masks
     META_SLOTS = 8
 };
@@ -104,7 +104,7 @@ struct PairCtx {
     int mask_mode;
     const int* cell_start;
     const int* meta;
-    int* queue;               // &meta[META_QUEUE]
+    int* queue;               // NQUEUE tile-queue heads, QSTRIDE ints apart
     double* cols;             // current column buffer
     const int* refidx;
     const double* prho2;      // P/rho^2 per particle (force pass)
@@ -116,6 +116,14 @@ struct PairCtx {
     int overwrite;            // write instead of accumulate
     int only_flagged;         // skip systems whose pass_on is 0
 };
+
+// Tile queues of the pair kernels: the tiles are split into NQUEUE
+// contiguous ranges with one head counter each (on its own 128-byte line);
+// workgroup b starts on queue b % NQUEUE (the XCD it runs on) and takes
+// QGRAB tiles per atomic, then helps the other queues.
+static constexpr int NQUEUE = 8;
+static constexpr int QSTRIDE = 32;
+static constexpr int QGRAB = 4;
 
 // Binary operator ids inside the library.
 enum PairOp {
@@ -136,6 +144,32 @@ struct Seg {
     double sx, sy, sz;
 };
 
+// Load through the constant address space: for tables a kernel only reads
+// (descriptors, cell starts, tiles), so that wave-uniform loads become
+// scalar loads the compiler may repeat instead of keeping the values live in
+// vector registers.
+#define SPH_CONST_AS __attribute__((address_space(4)))
+template <class T>
+__device__ inline T ld_const(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "ld_const copies 32-bit words");
+    T v;
+    const SPH_CONST_AS unsigned* src = (const SPH_CONST_AS unsigned*)p;
+    unsigned* dst = reinterpret_cast<unsigned*>(&v);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return v;
+}
+
+// x + s for periodic systems (s: image shift); x itself otherwise, where
+// the shift is zero and x + 0.0 would only turn -0.0 into +0.0.
+template <bool PERIODIC>
+__device__ inline double shifted(double x, double s)
+{
+    if constexpr (PERIODIC) return x + s;
+    else return x;
+}
+
 // The stencil rows of local cell `lc` of system `d`.
 //
 // Reference systems: row r = (dj, dk) holds the cells
@@ -148,7 +182,9 @@ struct Seg {
 // minimum-image shift; a row splits into at most two ranges at the x seam.
 // Every periodic axis has at least 3 cells, so no cell is visited twice.
 // Returns the number of ranges (0..2) written to seg.
-template <int ROWS>
+// PERIODIC = false compiles the reference rows only (a batch without
+// periodic systems).
+template <int ROWS, bool PERIODIC = true>
 __device__ inline int row_segments(const SysDesc& d, const int* cell_start, int lc, int row,
                                    Seg* seg)
 {
@@ -156,14 +192,14 @@ __device__ inline int row_segments(const SysDesc& d, const int* cell_start, int 
     const int dj = row % 3 - 1;
     const int dk = ROWS == 3 ? 0 : row / 3 - 1;
     const int base = d.cell_base;
-    if (!g.periodic) {
+    if (!PERIODIC || !g.periodic) {
         const long long mid = (long long)lc + (long long)g.L1 * dj + (long long)g.L1 * g.L2 * dk;
         long long lo = mid - 1, hi = mid + 1;
         if (lo < 0) lo = 0;
         if (hi > g.key_max - 1) hi = g.key_max - 1;
         if (lo > hi) return 0;
-        seg[0].j0 = cell_start[base + lo];
-        seg[0].j1 = cell_start[base + hi + 1];
+        seg[0].j0 = ld_const(cell_start + base + lo);
+        seg[0].j1 = ld_const(cell_start + base + hi + 1);
         seg[0].sx = seg[0].sy = seg[0].sz = 0.0;
         return seg[0].j0 < seg[0].j1 ? 1 : 0;
     }
@@ -183,25 +219,25 @@ __device__ inline int row_segments(const SysDesc& d, const int* cell_start, int 
     int lo = ci - 1;
     const int hi = ci + 1;
     if (lo < 0) {  // cell L1-1 seen through the left seam
-        const int a = cell_start[rowbase + g.L1 - 1], b = cell_start[rowbase + g.L1];
+        const int a = ld_const(cell_start + rowbase + g.L1 - 1), b = ld_const(cell_start + rowbase + g.L1);
         if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = -g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
         lo = 0;
     }
     const int hi_in = hi >= g.L1 ? g.L1 - 1 : hi;
     {
-        const int a = cell_start[rowbase + lo], b = cell_start[rowbase + hi_in + 1];
+        const int a = ld_const(cell_start + rowbase + lo), b = ld_const(cell_start + rowbase + hi_in + 1);
         if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = 0.0; seg[n].sy = sy; seg[n].sz = sz; ++n; }
     }
     if (hi >= g.L1) {  // cell 0 seen through the right seam
-        const int a = cell_start[rowbase], b = cell_start[rowbase + 1];
+        const int a = ld_const(cell_start + rowbase), b = ld_const(cell_start + rowbase + 1);
         if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
     }
     return n;
 }
 
 // Kernel launchers (implemented in the .hip files).
-void launch_pair(int op, int precision, int rows, int flat, const PairCtx& c, int grid,
-                 hipStream_t st);
+void launch_pair(int op, int precision, int rows, int flat, int periodic, const PairCtx& c,
+                 int grid, hipStream_t st);
 void launch_candidates(int rows, const PairCtx& c, const int* key, unsigned long long* out,
                        long long n, hipStream_t st);
 
@@ -214,6 +250,7 @@ struct sph_system {
     int tile_auto;       // choose tile_T from occupancy at each build
     int rows;            // 3 (2D) or 9 (3D): the same for every system of a batch
     int flat;            // every system is flat 2D
+    int periodic;        // some system is periodic
     long long n;         // host mirror of meta[META_N]
     long long cap;
     bool cells_valid;
@@ -238,6 +275,7 @@ struct sph_system {
 
     int* meta;           // device counters
     int* meta_host;      // pinned mirror
+    int* queue;          // NQUEUE * QSTRIDE ints: tile-queue heads
     unsigned long long* pair_counter;
 
     // build scratch
