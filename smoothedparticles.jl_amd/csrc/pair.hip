@@ -34,11 +34,11 @@
 // monotonic, so this equals s <= s_thr with s_thr = max{s : RN(sqrt(s)) <= h}
 // (computed on the host).  The FP64 kernels evaluate s without FMA
 // (-ffp-contract=off), i.e. bit-exactly as the reference.  The mixed
-// kernels test |q|^2 - 2 p.q <= s_thr (1 + BAND) - |p|^2 in FP32 on
-// tile-relative coordinates (three FMAs per candidate), and every candidate
-// accepted there whose direct FP32 squared distance lies above
-// s_thr (1 - BAND) is decided by the exact FP64 test, so their neighbour
-// sets (and masks) are the reference's too.  The identity exclusion p != q
+// kernels evaluate |q|^2 - 2 p.q + |p|^2 in FP32 on tile-relative
+// coordinates (three FMAs per candidate): below s_thr (1 - BAND) the pair is
+// accepted, above s_thr (1 + BAND) rejected, and in between the exact FP64
+// test decides, so their neighbour sets (and masks) are the reference's
+// too.  The identity exclusion p != q
 // is applied once per chunk to the slots that hold p.
 #include <type_traits>
 
@@ -244,6 +244,131 @@ __device__ inline void store_mask(const PairCtx& c, long long mbase, int chunk, 
     if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
 }
 
+// ------------------------------------------------------ balanced pairs --
+// The accepted pairs of a chunk are spread over all 64 lanes instead of
+// being walked by the lane that owns the target: with ~15% acceptance the
+// per-lane counts of a chunk differ a lot (targets near the chunk's
+// particles accept many, far ones none), and an owner-walked loop runs as
+// long as the busiest lane.  Per chunk:
+//   - the owner lane of target t (slice 0, lane t) publishes the target's
+//     64-bit word of accepted slots in LDS;
+//   - the pairs are cut into runs of at most B pairs, one target per run,
+//     with B the smallest quota (from the mean up) for which the runs fit
+//     in 64 lanes; lane l takes run l;
+//   - a lane reads its target's data from the owner lane (shuffles), skips
+//     to its run with a popcount descent and walks the run;
+//   - its partial sums go to LDS, and the owner folds the runs of its target
+//     in run order, so the result does not depend on timing.
+// The words are exact (the mixed test settles its guard band before), so
+// the runs -- and the summation order -- are the same whether a pass tests
+// the candidates or loads the density pass's words.
+template <class Part>
+struct Bal {
+    unsigned long long w[64];  // accepted slots of each target
+    int lane_t[64];            // target of each lane's run (-1: none)
+    Part part[64];             // each lane's partial sums of the chunk
+};
+
+// Bit b of a target's word is slot (b % T) * (64 / T) + b / T of the chunk.
+template <int T>
+__device__ inline int slot_of_bit(int b)
+{
+    return (b % T) * (64 / T) + b / T;
+}
+
+// w without its n lowest set bits (0 <= n < popcount(w)).
+__device__ inline unsigned long long drop_low_bits(unsigned long long w, int n)
+{
+    if (n <= 0) return w;
+    unsigned long long x = w;
+    int pos = 0;
+    int c = __popc((unsigned)x);
+    if (n >= c) { n -= c; x >>= 32; pos += 32; }
+    c = __popc((unsigned)x & 0xffffu);
+    if (n >= c) { n -= c; x >>= 16; pos += 16; }
+    c = __popc((unsigned)x & 0xffu);
+    if (n >= c) { n -= c; x >>= 8; pos += 8; }
+    c = __popc((unsigned)x & 0xfu);
+    if (n >= c) { n -= c; x >>= 4; pos += 4; }
+    c = __popc((unsigned)x & 0x3u);
+    if (n >= c) { n -= c; x >>= 2; pos += 2; }
+    c = (int)((unsigned)x & 1u);
+    if (n >= c) pos += 1;
+    return w & (~0ull << pos);
+}
+
+__device__ inline int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ inline int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Copy of v as held by lane src.
+template <class V>
+__device__ inline V shfl_value(const V& v, int src)
+{
+    static_assert(sizeof(V) % 4 == 0, "shuffled per 32-bit word");
+    V r;
+    const int* a = reinterpret_cast<const int*>(&v);
+    int* b = reinterpret_cast<int*>(&r);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(V) / 4; ++i) b[i] = __shfl(a[i], src);
+    return r;
+}
+
+// Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
+// only, 0 elsewhere).  Returns the total; sets the lane's run (target tt,
+// first rank r0, count cnt) and, for owner lanes, the first run and the
+// number of runs of their target.
+template <class Part>
+__device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int& r0, int& cnt,
+                                 int& first_run, int& nruns)
+{
+    const int tot = wave_sum(pop);
+    tt = -1;
+    r0 = cnt = first_run = nruns = 0;
+    if (tot == 0) return 0;
+    int B = (tot + 63) / 64;
+    int need = wave_sum(pop > 0 ? (pop + B - 1) / B : 0);
+    if (need > 64) {
+        ++B;
+        need = wave_sum(pop > 0 ? (pop + B - 1) / B : 0);
+        if (need > 64) {
+            const int nz = __popcll(__ballot(pop > 0));
+            B = nz < 64 ? (tot + (64 - nz) - 1) / (64 - nz) : wave_max(pop);
+        }
+    }
+    nruns = pop > 0 ? (pop + B - 1) / B : 0;
+    int incl = nruns;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    first_run = incl - nruns;
+    bal.lane_t[lane] = -1;
+    __syncthreads();
+    for (int k = 0; k < nruns; ++k) bal.lane_t[first_run + k] = lane;
+    __syncthreads();
+    tt = bal.lane_t[lane];
+    const int src = tt < 0 ? 0 : tt;
+    const int st = __shfl(first_run, src);
+    const int pt = __shfl(pop, src);
+    if (tt >= 0) {
+        r0 = (lane - st) * B;
+        cnt = min(B, pt - r0);
+    }
+    return tot;
+}
+
 // ================================================================ FP64 ====
 
 // LDS staging of one 64-candidate chunk (FP64): xy, (z, rho), (vx, vy),
@@ -289,6 +414,9 @@ struct Density64 {
         stage_pos64<P>(c, q, l, g, s);
     }
     static __device__ void init(Acc& a) { a.w = 0.0; }
+    using Part = Acc;
+    static __device__ void zero(Part& a) { a.w = 0.0; }
+    static __device__ void fold(Acc& a, const Part& q) { a.w += q.w; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds64&,
                                     int, int, int, double, double, double, double ss, Acc& a)
     {
@@ -342,6 +470,12 @@ struct Force64 {
         s.v2a[l] = make_double2(COL(c, C_VZ, q), c.prho2[q]);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
+    using Part = Acc;
+    static __device__ void zero(Part& a) { init(a); }
+    static __device__ void fold(Acc& a, const Part& q)
+    {
+        a.ax += q.ax; a.ay += q.ay; a.az += q.az; a.lx += q.lx; a.ly += q.ly; a.lz += q.lz;
+    }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds64& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -405,6 +539,9 @@ struct ForceRho064 {
         s.v2a[l].y = COL(c, C_P, q);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
+    using Part = Acc;
+    static __device__ void zero(Part& a) { init(a); }
+    static __device__ void fold(Acc& a, const Part& q) { a.ax += q.ax; a.ay += q.ay; a.az += q.az; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds64& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -457,6 +594,9 @@ struct Continuity64 {
         s.v2a[l].x = COL(c, C_VZ, q);
     }
     static __device__ void init(Acc& a) { a.d = 0.0; }
+    using Part = Acc;
+    static __device__ void zero(Part& a) { a.d = 0.0; }
+    static __device__ void fold(Acc& a, const Part& q) { a.d += q.d; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds64& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -499,6 +639,9 @@ struct Pairs64 {
         stage_pos64<P>(c, q, l, g, s);
     }
     static __device__ void init(Acc&) {}
+    using Part = Acc;
+    static __device__ void zero(Part&) {}
+    static __device__ void fold(Acc&, const Part&) {}
     static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds64&,
                                     int, int p, int q, double, double, double, double, Acc&)
     {
@@ -514,7 +657,8 @@ struct Pairs64 {
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
-                              Lds64& lds, Cands<ROWS>& L)
+                              Lds64& lds, Cands<ROWS>& L, Bal<typename Op::Part>& bal,
+                              unsigned& npairs)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -537,7 +681,6 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
-    unsigned npairs = 0;
     int cur = 0, chunk = 0;
     for (int base = 0; base < total; base += 64, ++chunk) {
         const int g = base + lane;
@@ -552,10 +695,12 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             pad_pos64(lane, lds);
         }
         __syncthreads();
-        MaskT<T> m = 0;
+        // accepted-slot word of target t (owner lanes publish it)
+        unsigned long long word = 0;
         if (load_masks) {
-            if (act) m = load_mask<T>(c, mbase, chunk, t, s);
+            if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
         } else {
+            MaskT<T> m = 0;
 #pragma unroll 4
             for (int j = 0; j < T; ++j) {
                 const int jj = s + (64 / T) * j;
@@ -570,28 +715,42 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             }
             if (!act) m = 0;
             exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
-            if (store_masks) store_mask<T>(c, mbase, chunk, t, s, m);
+            word = (unsigned long long)m << (T == 64 ? 0 : s * T);
+#pragma unroll
+            for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
+            if (s != 0) word = 0;
+            if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
         }
-        npairs += (unsigned)bit_count<T>(m);
-        if (Op::INTERACT) {
-            while (__any(m != 0)) {
-                if (m) {
-                    const int j = lowest_bit<T>(m);
-                    m &= m - 1;
-                    const int jj = s + (64 / T) * j;
+        const int pop = __popcll(word);
+        npairs += (unsigned)pop;
+        if (s == 0) bal.w[t] = word;
+        int tt, r0, cnt, frun, nrun;
+        const int tot = split_runs(lane, pop, bal, tt, r0, cnt, frun, nrun);
+        if (tot > 0) {
+            typename Op::Part part;
+            Op::zero(part);
+            const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
+            if (Op::INTERACT && cnt > 0) {
+                unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                for (int k = 0; k < cnt; ++k) {
+                    const int b = __ffsll((long long)w) - 1;
+                    w &= w - 1;
+                    const int jj = slot_of_bit<T>(b);
                     const double2 xy = lds.xy[jj];
-                    const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                    const double dz = FLAT ? 0.0 : tg.z - lds.zr[jj].x;
+                    const double dx = ht.x - xy.x, dy = ht.y - xy.y;
+                    const double dz = FLAT ? 0.0 : ht.z - lds.zr[jj].x;
                     double ss = dx * dx + dy * dy;
                     if (!FLAT) ss = ss + dz * dz;
-                    Op::interact(c, d, tg, lds, jj, p, L.q[jj], dx, dy, dz, ss, acc);
+                    Op::interact(c, d, ht, lds, jj, first + tt, L.q[jj], dx, dy, dz, ss, part);
                 }
             }
+            bal.part[lane] = part;
+            __syncthreads();
+            for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
         }
         cur = advance_range(L, nr, cur, base + 64);
         __syncthreads();
     }
-    if (c.pair_counter) count_pairs(c, npairs);
     Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
 
@@ -600,14 +759,18 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 {
     __shared__ Lds64 lds;
     __shared__ Cands<ROWS> L;
+    __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     TileCursor tc;
     tile_cursor_init(tc);
+    // accepted pairs of all the wave's tiles, one atomic per wave
+    unsigned npairs = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
-        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L);
+        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L, bal, npairs);
+    if (c.pair_counter) count_pairs(c, npairs);
 }
 
 // =============================================================== mixed ====
@@ -830,7 +993,8 @@ struct Pairs32 {
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, Lds32& lds, Cands<ROWS>& L)
+                              bool store_masks, Lds32& lds, Cands<ROWS>& L,
+                              Bal<typename Op::Part>& bal, unsigned& npairs)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -852,17 +1016,20 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double s_thr = d.g.s_thr;
-    const float thr_lo = (float)(s_thr * (1.0 - BAND));
-    // expanded test: |q|^2 - 2 p.q <= s_thr (1 + BAND) - |p|^2
+    // expanded test: |q|^2 - 2 p.q <= s_thr (1 -+ BAND) - |p|^2 -- below the
+    // lower threshold the pair is in, above the upper one out, and in
+    // between the exact FP64 test decides
     const float tz = FLAT ? 0.0f : tg.z;
-    const float thr_p = (float)(s_thr * (1.0 + BAND)) - fmaf(tz, tz, fmaf(tg.y, tg.y, tg.x * tg.x));
+    const float p2 = fmaf(tz, tz, fmaf(tg.y, tg.y, tg.x * tg.x));
+    const float thr_in = (float)(s_thr * (1.0 - BAND)) - p2;
+    const float thr_out = (float)(s_thr * (1.0 + BAND)) - p2;
     const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
+    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
     int total;
     const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
-    unsigned npairs = 0;
     int cur = 0, chunk = 0;
     for (int base = 0; base < total; base += 64, ++chunk) {
         const int g = base + lane;
@@ -878,10 +1045,12 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             pad_pos32(lane, lds);
         }
         __syncthreads();
-        MaskT<T> m = 0;
+        // accepted-slot word of target t (owner lanes publish it)
+        unsigned long long word = 0;
         if (load_masks) {
-            if (act) m = load_mask<T>(c, mbase, chunk, t, s);
+            if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
         } else {
+            MaskT<T> m = 0, unsure = 0;
 #pragma unroll 4
             for (int j = 0; j < T; ++j) {
                 const float4 a = lds.pos[s + (64 / T) * j];
@@ -889,52 +1058,66 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 float e = fmaf(mx, a.x, q2);
                 e = fmaf(my, a.y, e);
                 if (!FLAT) e = fmaf(mz, a.z, e);
-                if (e <= thr_p) m |= MaskT<T>(1) << j;
+                if (e <= thr_in) m |= MaskT<T>(1) << j;
+                if (e <= thr_out) unsure |= MaskT<T>(1) << j;
             }
-            if (!act) m = 0;
-            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
-        }
-        // accepted candidates: the guard band is decided in FP64, then the
-        // pair arithmetic
-        MaskT<T> keep = m;
-        typename Op::Part part;
-        Op::zero(part);
-        while (__any(m != 0)) {
-            if (m) {
-                const int j = lowest_bit<T>(m);
-                m &= m - 1;
-                const int jj = s + (64 / T) * j;
-                const float4 a = lds.pos[jj];
-                const float dx = tg.x - a.x, dy = tg.y - a.y;
-                const float dz = FLAT ? 0.0f : tg.z - a.z;
-                const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                bool ok = true;
-                if (!load_masks && ss > thr_lo) {
-                    // exact FP64 predicate of the reference
+            unsure &= ~m;
+            if (!act) m = unsure = 0;
+            // the guard band: exact FP64 predicate of the reference (rare)
+            while (__any(unsure != 0)) {
+                if (unsure) {
+                    const int j = lowest_bit<T>(unsure);
+                    unsure &= unsure - 1;
+                    const int jj = s + (64 / T) * j;
                     const int q = L.q[jj];
                     Seg sh{};
                     if (PERIODIC) sh = range_shift(L, L.rg[jj]);
-                    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp);
                     const double e1 = px - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
                     const double e2 = py - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
                     double es = e1 * e1 + e2 * e2;
                     if (!FLAT) {
-                        const double e3 = COL(c, C_Z, pp) - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
+                        const double e3 = pz - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
                         es = es + e3 * e3;
                     }
-                    ok = es <= s_thr;
-                    if (!ok) keep &= ~(MaskT<T>(1) << j);
+                    if (es <= s_thr) m |= MaskT<T>(1) << j;
                 }
-                if (Op::INTERACT && ok) Op::interact(c, d, tg, lds, jj, p, L.q[jj], dx, dy, dz, ss, part);
             }
+            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
+            word = (unsigned long long)m << (T == 64 ? 0 : s * T);
+#pragma unroll
+            for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
+            if (s != 0) word = 0;
         }
-        Op::fold(acc, part);
-        npairs += (unsigned)bit_count<T>(keep);
-        if (store_masks) store_mask<T>(c, mbase, chunk, t, s, keep);
+        if (s == 0) bal.w[t] = word;
+        int tt, r0, cnt, frun, nrun;
+        const int tot = split_runs(lane, __popcll(word), bal, tt, r0, cnt, frun, nrun);
+        if (tot > 0) {
+            typename Op::Part part;
+            Op::zero(part);
+            const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
+            if (Op::INTERACT && cnt > 0) {
+                unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                const int hp = first + tt;
+                for (int k = 0; k < cnt; ++k) {
+                    const int b = __ffsll((long long)w) - 1;
+                    w &= w - 1;
+                    const int jj = slot_of_bit<T>(b);
+                    const float4 a = lds.pos[jj];
+                    const float dx = ht.x - a.x, dy = ht.y - a.y;
+                    const float dz = FLAT ? 0.0f : ht.z - a.z;
+                    const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    Op::interact(c, d, ht, lds, jj, hp, L.q[jj], dx, dy, dz, ss, part);
+                }
+            }
+            bal.part[lane] = part;
+            __syncthreads();
+            for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
+        }
+        npairs += (unsigned)__popcll(word);
+        if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
         cur = advance_range(L, nr, cur, base + 64);
         __syncthreads();
     }
-    if (c.pair_counter) count_pairs(c, npairs);
     Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
@@ -943,14 +1126,19 @@ __global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
 {
     __shared__ Lds32 lds;
     __shared__ Cands<ROWS> L;
+    __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     TileCursor tc;
     tile_cursor_init(tc);
+    // accepted pairs of all the wave's tiles, one atomic per wave
+    unsigned npairs = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
-        tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L);
+        tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L, bal,
+                                            npairs);
+    if (c.pair_counter) count_pairs(c, npairs);
 }
 
 // ========================================================= diagnostics ====
