@@ -372,7 +372,8 @@ __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__
         for (int row = 0; row < ROWS; ++row) {
             Seg seg[2];
             const int ns = row_segments<ROWS>(d, cell_start, lc, row, seg);
-            for (int q = 0; q < ns; ++q) n += seg[q].j1 - seg[q].j0;
+            for (int q = 0; q < 2; ++q)
+                if (q < ns) n += seg[q].j1 - seg[q].j0;
         }
     }
     chunks[c] = (n + 63) / 64;

@@ -994,7 +994,8 @@ struct Pairs32 {
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
                               bool store_masks, Lds32& lds, Cands<ROWS>& L,
-                              Bal<typename Op::Part>& bal, unsigned& npairs)
+                              Bal<typename Op::Part>& bal, typename Op::Tgt* tgts,
+                              unsigned& npairs)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -1024,7 +1025,8 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const float thr_in = (float)(s_thr * (1.0 - BAND)) - p2;
     const float thr_out = (float)(s_thr * (1.0 + BAND)) - p2;
     const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
-    const double px = COL(c, C_X, pp), py = COL(c, C_Y, pp), pz = COL(c, C_Z, pp);
+    // the targets' records, read by the lanes that walk their pairs
+    if (s == 0) tgts[t] = tg;
     int total;
     const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     __syncthreads();
@@ -1072,11 +1074,11 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                     const int q = L.q[jj];
                     Seg sh{};
                     if (PERIODIC) sh = range_shift(L, L.rg[jj]);
-                    const double e1 = px - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
-                    const double e2 = py - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
+                    const double e1 = COL(c, C_X, pp) - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
+                    const double e2 = COL(c, C_Y, pp) - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
                     double es = e1 * e1 + e2 * e2;
                     if (!FLAT) {
-                        const double e3 = pz - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
+                        const double e3 = COL(c, C_Z, pp) - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
                         es = es + e3 * e3;
                     }
                     if (es <= s_thr) m |= MaskT<T>(1) << j;
@@ -1094,8 +1096,8 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
         if (tot > 0) {
             typename Op::Part part;
             Op::zero(part);
-            const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
             if (Op::INTERACT && cnt > 0) {
+                const typename Op::Tgt ht = tgts[tt];
                 unsigned long long w = drop_low_bits(bal.w[tt], r0);
                 const int hp = first + tt;
                 for (int k = 0; k < cnt; ++k) {
@@ -1127,6 +1129,7 @@ __global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
     __shared__ Lds32 lds;
     __shared__ Cands<ROWS> L;
     __shared__ Bal<typename Op::Part> bal;
+    __shared__ typename Op::Tgt tgts[T];
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
@@ -1137,7 +1140,7 @@ __global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
     unsigned npairs = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
         tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L, bal,
-                                            npairs);
+                                            tgts, npairs);
     if (c.pair_counter) count_pairs(c, npairs);
 }
 
@@ -1158,7 +1161,8 @@ __global__ __launch_bounds__(256) void k_candidates(PairCtx c, const int* __rest
         for (int row = 0; row < ROWS; ++row) {
             Seg segs[2];
             const int ns = row_segments<ROWS>(d, c.cell_start, cell - d.cell_base, row, segs);
-            for (int q = 0; q < ns; ++q) all += (unsigned long long)(segs[q].j1 - segs[q].j0);
+            for (int q = 0; q < 2; ++q)
+                if (q < ns) all += (unsigned long long)(segs[q].j1 - segs[q].j0);
         }
         if (COL(c, C_TYPE, p) == 0.0) fluid = all;
     }

@@ -215,24 +215,30 @@ __device__ inline int row_segments(const SysDesc& d, const int* cell_start, int 
         else if (k >= g.L3) { k -= g.L3; sz = g.box[2]; }
     }
     const int rowbase = base + g.L1 * (j + g.L2 * k);
-    int n = 0;
-    int lo = ci - 1;
-    const int hi = ci + 1;
-    if (lo < 0) {  // cell L1-1 seen through the left seam
-        const int a = ld_const(cell_start + rowbase + g.L1 - 1), b = ld_const(cell_start + rowbase + g.L1);
-        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = -g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
-        lo = 0;
+    // L1 >= 3: at most one of the two seams is crossed.  The outputs are
+    // written with fixed indices (selects), so they stay in registers.
+    const bool left = ci == 0, right = ci == g.L1 - 1;
+    const int lo = left ? 0 : ci - 1;
+    const int hi = right ? g.L1 - 1 : ci + 1;
+    Seg m, q;
+    m.j0 = ld_const(cell_start + rowbase + lo);
+    m.j1 = ld_const(cell_start + rowbase + hi + 1);
+    m.sx = 0.0; m.sy = sy; m.sz = sz;
+    q.j0 = q.j1 = 0;
+    q.sx = 0.0; q.sy = sy; q.sz = sz;
+    if (left) {  // cell L1-1 seen through the left seam
+        q.j0 = ld_const(cell_start + rowbase + g.L1 - 1);
+        q.j1 = ld_const(cell_start + rowbase + g.L1);
+        q.sx = -g.box[0];
+    } else if (right) {  // cell 0 seen through the right seam
+        q.j0 = ld_const(cell_start + rowbase);
+        q.j1 = ld_const(cell_start + rowbase + 1);
+        q.sx = g.box[0];
     }
-    const int hi_in = hi >= g.L1 ? g.L1 - 1 : hi;
-    {
-        const int a = ld_const(cell_start + rowbase + lo), b = ld_const(cell_start + rowbase + hi_in + 1);
-        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = 0.0; seg[n].sy = sy; seg[n].sz = sz; ++n; }
-    }
-    if (hi >= g.L1) {  // cell 0 seen through the right seam
-        const int a = ld_const(cell_start + rowbase), b = ld_const(cell_start + rowbase + 1);
-        if (a < b) { seg[n].j0 = a; seg[n].j1 = b; seg[n].sx = g.box[0]; seg[n].sy = sy; seg[n].sz = sz; ++n; }
-    }
-    return n;
+    const bool has_m = m.j0 < m.j1, has_q = q.j0 < q.j1;
+    seg[0] = has_m ? m : q;
+    seg[1] = q;
+    return (int)has_m + (int)has_q;
 }
 
 // Kernel launchers (implemented in the .hip files).
