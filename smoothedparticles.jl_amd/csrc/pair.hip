@@ -114,6 +114,29 @@ __device__ inline long long mask_base(const int4& td)
     return (long long)(unsigned)td.z | ((long long)td.w << 32);
 }
 
+// m * 2 + (e <= thr): the predicate shifted into a mask, as one compare
+// into VCC and one add with carry (the compiler would spend a select, a
+// shift and an OR per bit).  NaN compares false, as e <= thr does.
+__device__ inline unsigned shift_in_le(unsigned m, float e, float thr)
+{
+    unsigned r;
+    asm("v_cmp_ge_f32_e32 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+        : "=v"(r)
+        : "v"(m), "v"(thr), "v"(e)
+        : "vcc");
+    return r;
+}
+
+__device__ inline unsigned shift_in_le(unsigned m, double e, double thr)
+{
+    unsigned r;
+    asm("v_cmp_ge_f64_e32 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+        : "=v"(r)
+        : "v"(m), "v"(thr), "v"(e)
+        : "vcc");
+    return r;
+}
+
 // Per-lane mask of T slots.
 template <int T>
 using MaskT = typename std::conditional<(T > 32), unsigned long long, unsigned>::type;
@@ -700,18 +723,24 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
         if (load_masks) {
             if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
         } else {
+            // 32-bit words, slots shifted in from the highest bit down
             MaskT<T> m = 0;
+#pragma unroll
+            for (int h = 0; h < (T + 31) / 32; ++h) {
+                unsigned mh = 0;
 #pragma unroll 4
-            for (int j = 0; j < T; ++j) {
-                const int jj = s + (64 / T) * j;
-                const double2 xy = lds.xy[jj];
-                const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                double ss = dx * dx + dy * dy;
-                if (!FLAT) {
-                    const double dz = tg.z - lds.zr[jj].x;
-                    ss = ss + dz * dz;
+                for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
+                    const int jj = s + (64 / T) * (32 * h + j);
+                    const double2 xy = lds.xy[jj];
+                    const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                    double ss = dx * dx + dy * dy;
+                    if (!FLAT) {
+                        const double dz = tg.z - lds.zr[jj].x;
+                        ss = ss + dz * dz;
+                    }
+                    mh = shift_in_le(mh, ss, s_thr);
                 }
-                if (ss <= s_thr) m |= MaskT<T>(1) << j;
+                m |= (MaskT<T>)mh << (32 * h);
             }
             if (!act) m = 0;
             exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
@@ -1052,16 +1081,23 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
         if (load_masks) {
             if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
         } else {
+            // 32-bit words, slots shifted in from the highest bit down
             MaskT<T> m = 0, unsure = 0;
+#pragma unroll
+            for (int h = 0; h < (T + 31) / 32; ++h) {
+                unsigned min = 0, mout = 0;
 #pragma unroll 4
-            for (int j = 0; j < T; ++j) {
-                const float4 a = lds.pos[s + (64 / T) * j];
-                const float q2 = Op::STAGE_Q2 ? a.w : fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x));
-                float e = fmaf(mx, a.x, q2);
-                e = fmaf(my, a.y, e);
-                if (!FLAT) e = fmaf(mz, a.z, e);
-                if (e <= thr_in) m |= MaskT<T>(1) << j;
-                if (e <= thr_out) unsure |= MaskT<T>(1) << j;
+                for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
+                    const float4 a = lds.pos[s + (64 / T) * (32 * h + j)];
+                    const float q2 = Op::STAGE_Q2 ? a.w : fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x));
+                    float e = fmaf(mx, a.x, q2);
+                    e = fmaf(my, a.y, e);
+                    if (!FLAT) e = fmaf(mz, a.z, e);
+                    min = shift_in_le(min, e, thr_in);
+                    mout = shift_in_le(mout, e, thr_out);
+                }
+                m |= (MaskT<T>)min << (32 * h);
+                unsure |= (MaskT<T>)mout << (32 * h);
             }
             unsure &= ~m;
             if (!act) m = unsure = 0;
