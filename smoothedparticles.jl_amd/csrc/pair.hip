@@ -13,8 +13,16 @@
 // often as the reference does.  Since the particles are cell-sorted, each
 // row is one contiguous particle range (periodic systems: at most two, with
 // an image shift each).  All targets of a tile share the cell, hence the
-// ranges: the tile's candidate list is the concatenation of its ranges,
-// streamed through LDS in chunks of 64 that may span several ranges.
+// ranges: the tile's candidate list is the concatenation of its ranges.
+//
+// Culling.  Only the candidates within reach of the tile -- squared
+// distance to the bounding box of its targets not above the predicate's
+// threshold -- can be accepted by any target, about 60% of the list in 3D.
+// The list is streamed in 64-candidate steps; each step appends its kept
+// candidates (ballot + lane rank) to a ring of 128 LDS slots, and every 64
+// kept candidates form a chunk.  The kept order is the list order, and the
+// box and the cull test are the same in every pass, so all passes see the
+// same chunks.
 //
 // Each chunk is handled in two phases:
 //   1. mask: lane (t, s) tests the chunk slots s + S*j (j < T) against
@@ -38,8 +46,12 @@
 // coordinates (three FMAs per candidate): below s_thr (1 - BAND) the pair is
 // accepted, above s_thr (1 + BAND) rejected, and in between the exact FP64
 // test decides, so their neighbour sets (and masks) are the reference's
-// too.  The identity exclusion p != q
-// is applied once per chunk to the slots that hold p.
+// too.  The box test never drops a candidate that the predicate accepts:
+// in FP64 each box distance component is a correctly rounded difference no
+// larger than the target's, and the mixed test uses the same guard band.
+// The identity exclusion p != q is applied once per chunk to the slots that
+// hold p: the tile's own targets are always kept, so each occurrence of the
+// tile's cell in the list puts them in consecutive slots.
 #include <type_traits>
 
 #include "sph_internal.h"
@@ -159,21 +171,26 @@ __device__ inline int bit_count(MaskT<T> m)
 // The candidate ranges of a tile: its cell's stencil rows, concatenated.
 // pref[i] candidates precede range i; a chunk's slot l holds candidate
 // base + l, whose range i satisfies pref[i] <= base + l < pref[i + 1].
-template <int ROWS>
+// The kept candidates are staged in a ring of 128 slots: q (and rg, for
+// periodic systems) of each slot, and the kept position of the first
+// target at each occurrence of the tile's cell.
+template <int ROWS, bool PER>
 struct Cands {
     static constexpr int MAXR = 2 * ROWS;
+    static constexpr int NSH = PER ? MAXR : 1;
     int j0[MAXR];
     int pref[MAXR + 1];
-    double sx[MAXR], sy[MAXR], sz[MAXR];  // image shift (periodic systems)
-    int q[64];                            // particle index of each staged slot
-    int rg[64];                           // range of each staged slot
+    double sx[NSH], sy[NSH], sz[NSH];  // image shift (periodic systems)
+    int q[128];                        // particle index of each ring slot
+    int rg[PER ? 128 : 1];             // range of each ring slot
+    int self0[MAXR];
 };
 
 // Uniform work (every lane computes the same values); lane 0 writes.
 // Returns the number of ranges; *total receives the candidate count.
 template <int ROWS, bool PERIODIC>
-__device__ inline int build_cands(const SysDesc& d, const int* cell_start, int lc, Cands<ROWS>& L,
-                                  int& total)
+__device__ inline int build_cands(const SysDesc& d, const int* cell_start, int lc,
+                                  Cands<ROWS, PERIODIC>& L, int& total)
 {
     const bool w = __lane_id() == 0;
     int n = 0, tot = 0;
@@ -185,7 +202,7 @@ __device__ inline int build_cands(const SysDesc& d, const int* cell_start, int l
                 if (w) {
                     L.j0[n] = segs[k].j0;
                     L.pref[n] = tot;
-                    if (PERIODIC) {
+                    if constexpr (PERIODIC) {
                         L.sx[n] = segs[k].sx;
                         L.sy[n] = segs[k].sy;
                         L.sz[n] = segs[k].sz;
@@ -203,41 +220,24 @@ __device__ inline int build_cands(const SysDesc& d, const int* cell_start, int l
 
 // Slot of chunk `base` that holds candidate g (g < total): its range and
 // particle index.  Ranges are scanned from `cur`, the range of slot 0.
-template <int ROWS>
-__device__ inline int slot_range(const Cands<ROWS>& L, int cur, int g)
+template <int ROWS, bool PER>
+__device__ inline int slot_range(const Cands<ROWS, PER>& L, int cur, int g)
 {
     int i = cur;
     while (L.pref[i + 1] <= g) ++i;
     return i;
 }
 
-// Clear the bits of the slots of this chunk that hold the target itself
-// (the reference skips p === q, src/core.jl:105).  A small grid may list
-// the target's cell in several rows; every occurrence is cleared.
-template <int ROWS, int T>
-__device__ inline void exclude_self(const Cands<ROWS>& L, int nr, int cur, int base, int p, int s,
-                                    MaskT<T>& m)
-{
-    for (int i = cur; i < nr && L.pref[i] < base + 64; ++i) {
-        const int off = p - L.j0[i];
-        if (off >= 0 && off < L.pref[i + 1] - L.pref[i]) {
-            constexpr int S = 64 / T;
-            const int slot = L.pref[i] + off - base;
-            if (slot >= 0 && slot < 64 && slot % S == s) m &= ~(MaskT<T>(1) << (slot / S));
-        }
-    }
-}
-
 // Range of the next chunk's slot 0.
-template <int ROWS>
-__device__ inline int advance_range(const Cands<ROWS>& L, int nr, int cur, int next_base)
+template <int ROWS, bool PER>
+__device__ inline int advance_range(const Cands<ROWS, PER>& L, int nr, int cur, int next_base)
 {
     while (cur < nr && L.pref[cur + 1] <= next_base) ++cur;
     return cur;
 }
 
 template <int ROWS>
-__device__ inline Seg range_shift(const Cands<ROWS>& L, int i)
+__device__ inline Seg range_shift(const Cands<ROWS, true>& L, int i)
 {
     Seg g;
     g.j0 = g.j1 = 0;
@@ -245,6 +245,126 @@ __device__ inline Seg range_shift(const Cands<ROWS>& L, int i)
     g.sy = L.sy[i];
     g.sz = L.sz[i];
     return g;
+}
+
+// Lanes below this one with their bit set in m.
+__device__ inline int lane_rank(unsigned long long m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Append the lanes' kept candidates to the ring (list order).  fill: kept so
+// far; nself: occurrences of the first target so far.  Returns the lane's
+// ring slot, -1 if its candidate is not kept.
+template <int ROWS, bool PER>
+__device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool keep, int q, int rg,
+                              int first)
+{
+    const unsigned long long km = __ballot(keep);
+    const unsigned long long sm = __ballot(keep && q == first);
+    const int pos = fill + lane_rank(km);
+    int r = -1;
+    if (keep) {
+        r = pos & 127;
+        L.q[r] = q;
+        if constexpr (PER) L.rg[r] = rg;
+        if (q == first) L.self0[nself + lane_rank(sm)] = pos;
+    }
+    fill += __popcll(km);
+    nself += __popcll(sm);
+    return r;
+}
+
+// Clear the bits of the chunk's slots (kept positions head ... head + 63)
+// that hold the target itself (the reference skips p === q,
+// src/core.jl:105): target t sits t slots after the first target at every
+// occurrence of the tile's cell.
+template <int ROWS, bool PER, int T>
+__device__ inline void drop_self(const Cands<ROWS, PER>& L, int nself, int head, int t, int s,
+                                 MaskT<T>& m)
+{
+    constexpr int S = 64 / T;
+    for (int k = 0; k < nself; ++k) {
+        const int l = L.self0[k] + t - head;
+        if (l >= 0 && l < 64 && l % S == s) m &= ~(MaskT<T>(1) << (l / S));
+    }
+}
+
+// Bounding box of the tile's valid targets (the same value in every lane).
+template <class V>
+struct Box {
+    V x0, x1, y0, y1, z0, z1;
+};
+
+__device__ inline float uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ inline double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <class V>
+__device__ inline V wave_minv(V v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return uniform(v);
+}
+
+template <class V>
+__device__ inline V wave_maxv(V v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return uniform(v);
+}
+
+template <class V, bool FLAT>
+__device__ inline Box<V> tile_box(bool valid, V x, V y, V z)
+{
+    const V big = V(1.0e30);
+    Box<V> b;
+    b.x0 = wave_minv(valid ? x : big);
+    b.x1 = wave_maxv(valid ? x : -big);
+    b.y0 = wave_minv(valid ? y : big);
+    b.y1 = wave_maxv(valid ? y : -big);
+    b.z0 = b.z1 = V(0);
+    if (!FLAT) {
+        b.z0 = wave_minv(valid ? z : big);
+        b.z1 = wave_maxv(valid ? z : -big);
+    }
+    return b;
+}
+
+// Squared distance of (x, y, z) to the box (0 inside), evaluated in the
+// predicate's operation order.
+template <bool FLAT>
+__device__ inline double box_dist2(const Box<double>& b, double x, double y, double z)
+{
+    const double dx = fmax(fmax(b.x0 - x, x - b.x1), 0.0);
+    const double dy = fmax(fmax(b.y0 - y, y - b.y1), 0.0);
+    double ss = dx * dx + dy * dy;
+    if (!FLAT) {
+        const double dz = fmax(fmax(b.z0 - z, z - b.z1), 0.0);
+        ss = ss + dz * dz;
+    }
+    return ss;
+}
+
+template <bool FLAT>
+__device__ inline float box_dist2(const Box<float>& b, float x, float y, float z)
+{
+    const float dx = fmaxf(fmaxf(b.x0 - x, x - b.x1), 0.0f);
+    const float dy = fmaxf(fmaxf(b.y0 - y, y - b.y1), 0.0f);
+    const float dz = FLAT ? 0.0f : fmaxf(fmaxf(b.z0 - z, z - b.z1), 0.0f);
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 // Load the stored mask word of target t for a chunk; slice s keeps its bits.
@@ -392,34 +512,74 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int
     return tot;
 }
 
-// ================================================================ FP64 ====
-
-// LDS staging of one 64-candidate chunk (FP64): xy, (z, rho), (vx, vy),
-// (vz, aux) where aux is P/rho^2 (force) or P (force with rho0).
-struct Lds64 {
-    double2 xy[64], zr[64], v01[64], v2a[64];
+// ------------------------------------------------------------- staging --
+// A candidate's fields as loaded from the columns (the image shift already
+// added to the position).  The kernels load the next chunk's candidates into
+// registers before working on the current chunk and write them to LDS at
+// the next iteration, so the loads' latency overlaps the tests and pairs.
+// Each operator fills (fetch) and writes (put) only the fields it uses.
+struct Raw {
+    double x, y, z, a0, a1, a2, a3, a4;
 };
 
-// Candidate positions are staged with the periodic image shift of their
-// range added.
-template <bool P>
-__device__ inline void stage_pos64(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+// Slots past the end of the list: far away, rejected by every test.
+__device__ inline Raw pad_raw()
 {
-    s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
-    s.zr[l].x = shifted<P>(COL(c, C_Z, q), g.sz);
+    Raw r{};
+    r.x = r.y = r.z = 1.0e30;
+    return r;
 }
 
-// Slots past the end of the list: infinitely far, rejected by the test.
-__device__ inline void pad_pos64(int l, Lds64& s)
+template <bool P>
+__device__ inline void fetch_pos(const PairCtx& c, int q, const Seg& g, Raw& r)
 {
-    s.xy[l] = make_double2(INFINITY, INFINITY);
-    s.zr[l].x = INFINITY;
+    r.x = shifted<P>(COL(c, C_X, q), g.sx);
+    r.y = shifted<P>(COL(c, C_Y, q), g.sy);
+    r.z = shifted<P>(COL(c, C_Z, q), g.sz);
 }
+
+// Fetch candidate g of the list (a pad past its end) into registers; cur is
+// the range of the chunk's first slot.  rq, rg: its particle index and range.
+template <int ROWS, bool PERIODIC, class Op>
+__device__ inline void fetch_slot(const PairCtx& c, const Cands<ROWS, PERIODIC>& L, int total,
+                                  int cur, int g, Raw& raw, int& rq, int& rg)
+{
+    if (g < total) {
+        const int i = slot_range(L, cur, g);
+        const int q = L.j0[i] + (g - L.pref[i]);
+        Seg sh{};
+        if constexpr (PERIODIC) sh = range_shift(L, i);
+        raw = Op::template fetch<PERIODIC>(c, q, sh);
+        rq = q;
+        rg = i;
+    } else {
+        raw = pad_raw();
+        rq = 0;
+        rg = 0;
+    }
+}
+
+// ================================================================ FP64 ====
+
+// LDS ring of kept candidates (FP64): positions only, or xy, (z, rho),
+// (vx, vy), (vz, aux) where aux is P/rho^2 (force) or P (force with rho0).
+struct Lds64Pos {
+    double2 xy[128];
+    double z[128];
+};
+
+struct Lds64Full {
+    double2 xy[128], zr[128], v01[128], v2a[128];
+};
+
+__device__ inline double lds_z(const Lds64Pos& s, int j) { return s.z[j]; }
+__device__ inline double lds_z(const Lds64Full& s, int j) { return s.zr[j].x; }
 
 // rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0).
 template <int OUT>
 struct Density64 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds64Pos;
     struct Tgt {
         double x, y, z;
     };
@@ -432,15 +592,22 @@ struct Density64 {
     }
     static __device__ bool active(const Tgt&) { return true; }
     template <bool P>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        stage_pos64<P>(c, q, l, g, s);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.z[l] = r.z;
     }
     static __device__ void init(Acc& a) { a.w = 0.0; }
     using Part = Acc;
     static __device__ void zero(Part& a) { a.w = 0.0; }
     static __device__ void fold(Acc& a, const Part& q) { a.w += q.w; }
-    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds64&,
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
                                     int, int, int, double, double, double, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -468,6 +635,7 @@ struct Density64 {
 //   laminar   a += 2 kdW mu / rho0^2 u^3 v_pq
 struct Force64 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -485,12 +653,21 @@ struct Force64 {
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
     template <bool P>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
-        s.zr[l] = make_double2(shifted<P>(COL(c, C_Z, q), g.sz), COL(c, C_RHO, q));
-        s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
-        s.v2a[l] = make_double2(COL(c, C_VZ, q), c.prho2[q]);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_RHO, q);
+        r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
+        r.a4 = c.prho2[q];
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.zr[l] = make_double2(r.z, r.a0);
+        s.v01[l] = make_double2(r.a1, r.a2);
+        s.v2a[l] = make_double2(r.a3, r.a4);
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
     using Part = Acc;
@@ -500,7 +677,7 @@ struct Force64 {
         a.ax += q.ax; a.ay += q.ay; a.az += q.az; a.lx += q.lx; a.ly += q.ly; a.lz += q.lz;
     }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -544,6 +721,7 @@ struct Force64 {
 // internal_force! of tests/test_collision_2d.jl:75-78 (P/rho0^2, all targets).
 struct ForceRho064 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, P;
     };
@@ -556,17 +734,25 @@ struct ForceRho064 {
     }
     static __device__ bool active(const Tgt&) { return true; }
     template <bool P>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        stage_pos64<P>(c, q, l, g, s);
-        s.v2a[l].y = COL(c, C_P, q);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_P, q);
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.zr[l].x = r.z;
+        s.v2a[l].y = r.a0;
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
     using Part = Acc;
     static __device__ void zero(Part& a) { init(a); }
     static __device__ void fold(Acc& a, const Part& q) { a.ax += q.ax; a.ay += q.ay; a.az += q.az; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -595,6 +781,7 @@ struct ForceRho064 {
 //   Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q))
 struct Continuity64 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, vx, vy, vz, rho;
     };
@@ -609,19 +796,27 @@ struct Continuity64 {
     }
     static __device__ bool active(const Tgt&) { return true; }
     template <bool P>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        s.xy[l] = make_double2(shifted<P>(COL(c, C_X, q), g.sx), shifted<P>(COL(c, C_Y, q), g.sy));
-        s.zr[l] = make_double2(shifted<P>(COL(c, C_Z, q), g.sz), COL(c, C_RHO, q));
-        s.v01[l] = make_double2(COL(c, C_VX, q), COL(c, C_VY, q));
-        s.v2a[l].x = COL(c, C_VZ, q);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_RHO, q);
+        r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.zr[l] = make_double2(r.z, r.a0);
+        s.v01[l] = make_double2(r.a1, r.a2);
+        s.v2a[l].x = r.a3;
     }
     static __device__ void init(Acc& a) { a.d = 0.0; }
     using Part = Acc;
     static __device__ void zero(Part& a) { a.d = 0.0; }
     static __device__ void fold(Acc& a, const Part& q) { a.d += q.d; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds64& s, int jj, int, int, double dx, double dy,
+                                    const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -645,6 +840,7 @@ struct Continuity64 {
 template <bool EMIT>
 struct Pairs64 {
     static constexpr bool INTERACT = EMIT;
+    using Lds = Lds64Pos;
     struct Tgt {
         double x, y, z;
     };
@@ -657,15 +853,22 @@ struct Pairs64 {
     }
     static __device__ bool active(const Tgt&) { return true; }
     template <bool P>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, Lds64& s)
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        stage_pos64<P>(c, q, l, g, s);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.z[l] = r.z;
     }
     static __device__ void init(Acc&) {}
     using Part = Acc;
     static __device__ void zero(Part&) {}
     static __device__ void fold(Acc&, const Part&) {}
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds64&,
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
                                     int, int p, int q, double, double, double, double, Acc&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
@@ -680,8 +883,8 @@ struct Pairs64 {
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
-                              Lds64& lds, Cands<ROWS>& L, Bal<typename Op::Part>& bal,
-                              unsigned& npairs)
+                              typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
+                              Bal<typename Op::Part>& bal, unsigned& npairs)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -690,6 +893,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
     const int cend = ld_const(c.cell_start + cell + 1);
+    const int nvalid = min(T, cend - first);
     const int lane = threadIdx.x;
     const int t = lane & (T - 1), s = lane / T;
     const int p = first + t;
@@ -699,86 +903,101 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double s_thr = d.g.s_thr;
+    const Box<double> box = tile_box<double, FLAT>(valid, tg.x, tg.y, tg.z);
     int total;
     const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
-    int cur = 0, chunk = 0;
-    for (int base = 0; base < total; base += 64, ++chunk) {
-        const int g = base + lane;
-        if (g < total) {
-            const int i = slot_range(L, cur, g);
-            const int q = L.j0[i] + (g - L.pref[i]);
-            Seg sh{};
-            if (PERIODIC) sh = range_shift(L, i);
-            Op::template stage<PERIODIC>(c, q, lane, sh, lds);
-            L.q[lane] = q;
-        } else {
-            pad_pos64(lane, lds);
-        }
+    int cur = 0, fill = 0, head = 0, nself = 0, chunk = 0;
+    for (int base = 0; base < total; base += 64) {
+        Raw raw;
+        int rq = 0, rrg = 0;
+        cur = advance_range(L, nr, cur, base);
+        fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
+        // keep the candidates within reach of the box, and the tile's targets
+        const bool keep = base + lane < total &&
+                          (box_dist2<FLAT>(box, raw.x, raw.y, raw.z) <= s_thr ||
+                           (unsigned)(rq - first) < (unsigned)nvalid);
+        const int r = compact(L, fill, nself, keep, rq, rrg, first);
+        if (r >= 0) Op::put(raw, r, lds);
+        const bool last = base + 64 >= total;
         __syncthreads();
-        // accepted-slot word of target t (owner lanes publish it)
-        unsigned long long word = 0;
-        if (load_masks) {
-            if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
-        } else {
-            // 32-bit words, slots shifted in from the highest bit down
-            MaskT<T> m = 0;
+        while (fill - head >= 64 || (last && fill > head)) {
+            if (fill - head < 64) {
+                // the last chunk: pad its free slots
+                const int k = head + lane;
+                if (k >= fill) {
+                    Op::put(pad_raw(), k & 127, lds);
+                    L.q[k & 127] = first;
+                }
+                __syncthreads();
+            }
+            const int rb = head & 64;  // ring half of the chunk
+            // accepted-slot word of target t (owner lanes publish it)
+            unsigned long long word = 0;
+            if (load_masks) {
+                if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
+            } else {
+                // 32-bit words, slots shifted in from the highest bit down
+                MaskT<T> m = 0;
 #pragma unroll
-            for (int h = 0; h < (T + 31) / 32; ++h) {
-                unsigned mh = 0;
+                for (int h = 0; h < (T + 31) / 32; ++h) {
+                    unsigned mh = 0;
 #pragma unroll 4
-                for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
-                    const int jj = s + (64 / T) * (32 * h + j);
-                    const double2 xy = lds.xy[jj];
-                    const double dx = tg.x - xy.x, dy = tg.y - xy.y;
-                    double ss = dx * dx + dy * dy;
-                    if (!FLAT) {
-                        const double dz = tg.z - lds.zr[jj].x;
-                        ss = ss + dz * dz;
+                    for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
+                        const int jj = rb + s + (64 / T) * (32 * h + j);
+                        const double2 xy = lds.xy[jj];
+                        const double dx = tg.x - xy.x, dy = tg.y - xy.y;
+                        double ss = dx * dx + dy * dy;
+                        if (!FLAT) {
+                            const double dz = tg.z - lds_z(lds, jj);
+                            ss = ss + dz * dz;
+                        }
+                        mh = shift_in_le(mh, ss, s_thr);
                     }
-                    mh = shift_in_le(mh, ss, s_thr);
+                    m |= (MaskT<T>)mh << (32 * h);
                 }
-                m |= (MaskT<T>)mh << (32 * h);
-            }
-            if (!act) m = 0;
-            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
-            word = (unsigned long long)m << (T == 64 ? 0 : s * T);
+                if (!act) m = 0;
+                drop_self<ROWS, PERIODIC, T>(L, nself, head, t, s, m);
+                word = (unsigned long long)m << (T == 64 ? 0 : s * T);
 #pragma unroll
-            for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
-            if (s != 0) word = 0;
-            if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
-        }
-        const int pop = __popcll(word);
-        npairs += (unsigned)pop;
-        if (s == 0) bal.w[t] = word;
-        int tt, r0, cnt, frun, nrun;
-        const int tot = split_runs(lane, pop, bal, tt, r0, cnt, frun, nrun);
-        if (tot > 0) {
-            typename Op::Part part;
-            Op::zero(part);
-            const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
-            if (Op::INTERACT && cnt > 0) {
-                unsigned long long w = drop_low_bits(bal.w[tt], r0);
-                for (int k = 0; k < cnt; ++k) {
-                    const int b = __ffsll((long long)w) - 1;
-                    w &= w - 1;
-                    const int jj = slot_of_bit<T>(b);
-                    const double2 xy = lds.xy[jj];
-                    const double dx = ht.x - xy.x, dy = ht.y - xy.y;
-                    const double dz = FLAT ? 0.0 : ht.z - lds.zr[jj].x;
-                    double ss = dx * dx + dy * dy;
-                    if (!FLAT) ss = ss + dz * dz;
-                    Op::interact(c, d, ht, lds, jj, first + tt, L.q[jj], dx, dy, dz, ss, part);
-                }
+                for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
+                if (s != 0) word = 0;
+                if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
             }
-            bal.part[lane] = part;
+            const int pop = __popcll(word);
+            npairs += (unsigned)pop;
+            if (s == 0) bal.w[t] = word;
+            int tt, r0, cnt, frun, nrun;
+            const int tot = split_runs(lane, pop, bal, tt, r0, cnt, frun, nrun);
+            if (tot > 0) {
+                typename Op::Part part;
+                Op::zero(part);
+                const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
+                if (Op::INTERACT && cnt > 0) {
+                    unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                    for (int k = 0; k < cnt; ++k) {
+                        const int b = __ffsll((long long)w) - 1;
+                        w &= w - 1;
+                        const int jj = rb + slot_of_bit<T>(b);
+                        const double2 xy = lds.xy[jj];
+                        const double dx = ht.x - xy.x, dy = ht.y - xy.y;
+                        const double dz = FLAT ? 0.0 : ht.z - lds_z(lds, jj);
+                        double ss = dx * dx + dy * dy;
+                        if (!FLAT) ss = ss + dz * dz;
+                        Op::interact(c, d, ht, lds, jj, first + tt, L.q[jj], dx, dy, dz, ss,
+                                     part);
+                    }
+                }
+                bal.part[lane] = part;
+                __syncthreads();
+                for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
+            }
+            ++chunk;
+            head += 64;
             __syncthreads();
-            for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
         }
-        cur = advance_range(L, nr, cur, base + 64);
-        __syncthreads();
     }
     Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
@@ -786,8 +1005,8 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 {
-    __shared__ Lds64 lds;
-    __shared__ Cands<ROWS> L;
+    __shared__ typename Op::Lds lds;
+    __shared__ Cands<ROWS, PERIODIC> L;
     __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
@@ -808,33 +1027,31 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 // partial sums are plain FP32 within a 64-candidate chunk and folded into a
 // compensated (sum, error) pair after every chunk.
 
-// LDS staging (FP32): (x, y, z, |x|^2) -- (x, y, z, P/rho^2) for the
-// force pass -- and, for the force pass, (vx, vy, vz, rho).
-struct Lds32 {
-    float4 pos[64], v[64];
+// LDS ring of kept candidates (FP32): (x, y, z, |x|^2) -- (x, y, z,
+// P/rho^2) for the force pass -- and, for the force pass, (vx, vy, vz, rho).
+struct Lds32Pos {
+    float4 pos[128];
 };
 
-// Relative position of candidate q (with its image shift) and |.|^2.
-template <bool P, bool FLAT>
-__device__ inline float4 rel_pos32(const PairCtx& c, int q, const Seg& g, double ox, double oy,
-                                   double oz)
-{
-    const float x = (float)(shifted<P>(COL(c, C_X, q), g.sx) - ox);
-    const float y = (float)(shifted<P>(COL(c, C_Y, q), g.sy) - oy);
-    const float z = FLAT ? 0.0f : (float)(shifted<P>(COL(c, C_Z, q), g.sz) - oz);
-    return make_float4(x, y, z, fmaf(z, z, fmaf(y, y, x * x)));
-}
+struct Lds32Full {
+    float4 pos[128], v[128];
+};
 
-// Slots past the end of the list fail every test: |x|^2 = inf, and the
-// coordinates are far away for operators that recompute |x|^2.
-__device__ inline void pad_pos32(int l, Lds32& s)
+// Tile-relative FP32 position of a fetched candidate and its |.|^2.
+template <bool FLAT>
+__device__ inline float4 rel_pos32(const Raw& r, double ox, double oy, double oz)
 {
-    s.pos[l] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, INFINITY);
+    const float x = (float)(r.x - ox);
+    const float y = (float)(r.y - oy);
+    const float z = FLAT ? 0.0f : (float)(r.z - oz);
+    return make_float4(x, y, z, fmaf(z, z, fmaf(y, y, x * x)));
 }
 
 struct Density32 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;  // pos.w = |x|^2
+    static constexpr int WAVES = 5;
     struct Tgt {
         float x, y, z;
     };
@@ -851,15 +1068,21 @@ struct Density32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    template <bool P, bool FLAT>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
-                                 double oy, double oz, Lds32& s)
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        s.pos[l] = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    {
+        s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
     static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
-    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds32&,
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
                                     int, int, int, float, float, float, float ss, Part& q)
     {
         const float r = __builtin_sqrtf(ss);
@@ -884,7 +1107,9 @@ struct Density32 {
 
 struct Force32 {
     static constexpr bool INTERACT = true;
+    using Lds = Lds32Full;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
+    static constexpr int WAVES = 4;
     struct Tgt {
         float x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -906,15 +1131,23 @@ struct Force32 {
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
-    template <bool P, bool FLAT>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
-                                 double oy, double oz, Lds32& s)
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        float4 a = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
-        a.w = (float)c.prho2[q];
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_RHO, q);
+        r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
+        r.a4 = c.prho2[q];
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    {
+        float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
+        a.w = (float)r.a4;
         s.pos[l] = a;
-        s.v[l] = make_float4((float)COL(c, C_VX, q), (float)COL(c, C_VY, q),
-                             (float)COL(c, C_VZ, q), (float)COL(c, C_RHO, q));
+        s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, (float)r.a0);
     }
     static __device__ void init(Acc& a)
     {
@@ -923,7 +1156,7 @@ struct Force32 {
     }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds32& s, int jj, int, int, float dx, float dy,
+                                    const Lds& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
         const float r = __builtin_sqrtf(ss);
@@ -981,7 +1214,9 @@ struct Force32 {
 template <bool EMIT>
 struct Pairs32 {
     static constexpr bool INTERACT = EMIT;
+    using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;
+    static constexpr int WAVES = 5;
     struct Tgt {
         float x, y, z;
     };
@@ -998,15 +1233,21 @@ struct Pairs32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
     }
     static __device__ bool active(const Tgt&) { return true; }
-    template <bool P, bool FLAT>
-    static __device__ void stage(const PairCtx& c, int q, int l, const Seg& g, double ox,
-                                 double oy, double oz, Lds32& s)
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
     {
-        s.pos[l] = rel_pos32<P, FLAT>(c, q, g, ox, oy, oz);
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    {
+        s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
     static __device__ void init(Acc&) {}
     static __device__ void zero(Part&) {}
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds32&,
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
                                     int, int p, int q, float, float, float, float, Part&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
@@ -1022,7 +1263,7 @@ struct Pairs32 {
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, Lds32& lds, Cands<ROWS>& L,
+                              bool store_masks, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
                               Bal<typename Op::Part>& bal, typename Op::Tgt* tgts,
                               unsigned& npairs)
 {
@@ -1033,6 +1274,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const int lc = cell - d.cell_base;
     const long long mbase = mask_base(td);
     const int cend = ld_const(c.cell_start + cell + 1);
+    const int nvalid = min(T, cend - first);
     const int lane = threadIdx.x;
     const int t = lane & (T - 1), s = lane / T;
     const int p = first + t;
@@ -1054,6 +1296,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const float thr_in = (float)(s_thr * (1.0 - BAND)) - p2;
     const float thr_out = (float)(s_thr * (1.0 + BAND)) - p2;
     const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
+    // the cull keeps everything within the guard band of the box
+    const Box<float> box = tile_box<float, FLAT>(valid, tg.x, tg.y, tz);
+    const float cull_thr = (float)(s_thr * (1.0 + BAND));
     // the targets' records, read by the lanes that walk their pairs
     if (s == 0) tgts[t] = tg;
     int total;
@@ -1061,109 +1306,128 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
-    int cur = 0, chunk = 0;
-    for (int base = 0; base < total; base += 64, ++chunk) {
-        const int g = base + lane;
-        if (g < total) {
-            const int i = slot_range(L, cur, g);
-            const int q = L.j0[i] + (g - L.pref[i]);
-            Seg sh{};
-            if (PERIODIC) sh = range_shift(L, i);
-            Op::template stage<PERIODIC, FLAT>(c, q, lane, sh, ox, oy, oz, lds);
-            L.q[lane] = q;
-            if (PERIODIC) L.rg[lane] = i;
-        } else {
-            pad_pos32(lane, lds);
-        }
+    int cur = 0, fill = 0, head = 0, nself = 0, chunk = 0;
+    for (int base = 0; base < total; base += 64) {
+        Raw raw;
+        int rq = 0, rrg = 0;
+        cur = advance_range(L, nr, cur, base);
+        fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
+        // keep the candidates within reach of the box, and the tile's targets
+        const float4 a = rel_pos32<FLAT>(raw, ox, oy, oz);
+        const bool keep = base + lane < total &&
+                          (box_dist2<FLAT>(box, a.x, a.y, a.z) <= cull_thr ||
+                           (unsigned)(rq - first) < (unsigned)nvalid);
+        const int r = compact(L, fill, nself, keep, rq, rrg, first);
+        if (r >= 0) Op::template put<FLAT>(raw, r, ox, oy, oz, lds);
+        const bool last = base + 64 >= total;
         __syncthreads();
-        // accepted-slot word of target t (owner lanes publish it)
-        unsigned long long word = 0;
-        if (load_masks) {
-            if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
-        } else {
-            // 32-bit words, slots shifted in from the highest bit down
-            MaskT<T> m = 0, unsure = 0;
+        while (fill - head >= 64 || (last && fill > head)) {
+            if (fill - head < 64) {
+                // the last chunk: pad its free slots
+                const int k = head + lane;
+                if (k >= fill) {
+                    Op::template put<FLAT>(pad_raw(), k & 127, ox, oy, oz, lds);
+                    L.q[k & 127] = first;
+                    if constexpr (PERIODIC) L.rg[k & 127] = 0;
+                }
+                __syncthreads();
+            }
+            const int rb = head & 64;  // ring half of the chunk
+            // accepted-slot word of target t (owner lanes publish it)
+            unsigned long long word = 0;
+            if (load_masks) {
+                if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
+            } else {
+                // 32-bit words, slots shifted in from the highest bit down
+                MaskT<T> m = 0, unsure = 0;
 #pragma unroll
-            for (int h = 0; h < (T + 31) / 32; ++h) {
-                unsigned min = 0, mout = 0;
+                for (int h = 0; h < (T + 31) / 32; ++h) {
+                    unsigned min = 0, mout = 0;
 #pragma unroll 4
-                for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
-                    const float4 a = lds.pos[s + (64 / T) * (32 * h + j)];
-                    const float q2 = Op::STAGE_Q2 ? a.w : fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x));
-                    float e = fmaf(mx, a.x, q2);
-                    e = fmaf(my, a.y, e);
-                    if (!FLAT) e = fmaf(mz, a.z, e);
-                    min = shift_in_le(min, e, thr_in);
-                    mout = shift_in_le(mout, e, thr_out);
-                }
-                m |= (MaskT<T>)min << (32 * h);
-                unsure |= (MaskT<T>)mout << (32 * h);
-            }
-            unsure &= ~m;
-            if (!act) m = unsure = 0;
-            // the guard band: exact FP64 predicate of the reference (rare)
-            while (__any(unsure != 0)) {
-                if (unsure) {
-                    const int j = lowest_bit<T>(unsure);
-                    unsure &= unsure - 1;
-                    const int jj = s + (64 / T) * j;
-                    const int q = L.q[jj];
-                    Seg sh{};
-                    if (PERIODIC) sh = range_shift(L, L.rg[jj]);
-                    const double e1 = COL(c, C_X, pp) - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
-                    const double e2 = COL(c, C_Y, pp) - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
-                    double es = e1 * e1 + e2 * e2;
-                    if (!FLAT) {
-                        const double e3 = COL(c, C_Z, pp) - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
-                        es = es + e3 * e3;
+                    for (int j = (T < 32 ? T : 32) - 1; j >= 0; --j) {
+                        const float4 a = lds.pos[rb + s + (64 / T) * (32 * h + j)];
+                        const float q2 =
+                            Op::STAGE_Q2 ? a.w : fmaf(a.z, a.z, fmaf(a.y, a.y, a.x * a.x));
+                        float e = fmaf(mx, a.x, q2);
+                        e = fmaf(my, a.y, e);
+                        if (!FLAT) e = fmaf(mz, a.z, e);
+                        min = shift_in_le(min, e, thr_in);
+                        mout = shift_in_le(mout, e, thr_out);
                     }
-                    if (es <= s_thr) m |= MaskT<T>(1) << j;
+                    m |= (MaskT<T>)min << (32 * h);
+                    unsure |= (MaskT<T>)mout << (32 * h);
                 }
-            }
-            exclude_self<ROWS, T>(L, nr, cur, base, p, s, m);
-            word = (unsigned long long)m << (T == 64 ? 0 : s * T);
+                unsure &= ~m;
+                if (!act) m = unsure = 0;
+                // the guard band: exact FP64 predicate of the reference (rare)
+                while (__any(unsure != 0)) {
+                    if (unsure) {
+                        const int j = lowest_bit<T>(unsure);
+                        unsure &= unsure - 1;
+                        const int jj = rb + s + (64 / T) * j;
+                        const int q = L.q[jj];
+                        Seg sh{};
+                        if constexpr (PERIODIC) sh = range_shift(L, L.rg[jj]);
+                        const double e1 =
+                            COL(c, C_X, pp) - shifted<PERIODIC>(COL(c, C_X, q), sh.sx);
+                        const double e2 =
+                            COL(c, C_Y, pp) - shifted<PERIODIC>(COL(c, C_Y, q), sh.sy);
+                        double es = e1 * e1 + e2 * e2;
+                        if (!FLAT) {
+                            const double e3 =
+                                COL(c, C_Z, pp) - shifted<PERIODIC>(COL(c, C_Z, q), sh.sz);
+                            es = es + e3 * e3;
+                        }
+                        if (es <= s_thr) m |= MaskT<T>(1) << j;
+                    }
+                }
+                drop_self<ROWS, PERIODIC, T>(L, nself, head, t, s, m);
+                word = (unsigned long long)m << (T == 64 ? 0 : s * T);
 #pragma unroll
-            for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
-            if (s != 0) word = 0;
-        }
-        if (s == 0) bal.w[t] = word;
-        int tt, r0, cnt, frun, nrun;
-        const int tot = split_runs(lane, __popcll(word), bal, tt, r0, cnt, frun, nrun);
-        if (tot > 0) {
-            typename Op::Part part;
-            Op::zero(part);
-            if (Op::INTERACT && cnt > 0) {
-                const typename Op::Tgt ht = tgts[tt];
-                unsigned long long w = drop_low_bits(bal.w[tt], r0);
-                const int hp = first + tt;
-                for (int k = 0; k < cnt; ++k) {
-                    const int b = __ffsll((long long)w) - 1;
-                    w &= w - 1;
-                    const int jj = slot_of_bit<T>(b);
-                    const float4 a = lds.pos[jj];
-                    const float dx = ht.x - a.x, dy = ht.y - a.y;
-                    const float dz = FLAT ? 0.0f : ht.z - a.z;
-                    const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    Op::interact(c, d, ht, lds, jj, hp, L.q[jj], dx, dy, dz, ss, part);
-                }
+                for (int o = T; o < 64; o <<= 1) word |= __shfl_xor(word, o);
+                if (s != 0) word = 0;
             }
-            bal.part[lane] = part;
+            if (s == 0) bal.w[t] = word;
+            int tt, r0, cnt, frun, nrun;
+            const int tot = split_runs(lane, __popcll(word), bal, tt, r0, cnt, frun, nrun);
+            if (tot > 0) {
+                typename Op::Part part;
+                Op::zero(part);
+                if (Op::INTERACT && cnt > 0) {
+                    const typename Op::Tgt ht = tgts[tt];
+                    unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                    const int hp = first + tt;
+                    for (int k = 0; k < cnt; ++k) {
+                        const int b = __ffsll((long long)w) - 1;
+                        w &= w - 1;
+                        const int jj = rb + slot_of_bit<T>(b);
+                        const float4 a = lds.pos[jj];
+                        const float dx = ht.x - a.x, dy = ht.y - a.y;
+                        const float dz = FLAT ? 0.0f : ht.z - a.z;
+                        const float ss =
+                            FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        Op::interact(c, d, ht, lds, jj, hp, L.q[jj], dx, dy, dz, ss, part);
+                    }
+                }
+                bal.part[lane] = part;
+                __syncthreads();
+                for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
+            }
+            npairs += (unsigned)__popcll(word);
+            if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
+            ++chunk;
+            head += 64;
             __syncthreads();
-            for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
         }
-        npairs += (unsigned)__popcll(word);
-        if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
-        cur = advance_range(L, nr, cur, base + 64);
-        __syncthreads();
     }
     Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
-__global__ __launch_bounds__(64) void k_pair32(PairCtx c, int out_col)
+__global__ __launch_bounds__(64, Op::WAVES) void k_pair32(PairCtx c, int out_col)
 {
-    __shared__ Lds32 lds;
-    __shared__ Cands<ROWS> L;
+    __shared__ typename Op::Lds lds;
+    __shared__ Cands<ROWS, PERIODIC> L;
     __shared__ Bal<typename Op::Part> bal;
     __shared__ typename Op::Tgt tgts[T];
     const int ntiles = ld_const(c.meta + META_NTILES);
