@@ -44,11 +44,13 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
 # Algorithmic work of the two pair passes of a step.
-#   density pass: every candidate of every target is tested once (3 sub,
-#     1 mul, 2 fma, 1 compare, 2 band compares = 9 flops), every accepted
-#     pair adds m W (about 15 flops); bytes: positions read and rho written
-#     once in sorted order (the accepted-candidate masks it writes are
-#     scratch, not algorithmic).
+#   density pass: the distance tests it performs (one bounding-box test per
+#     candidate per tile, one pair test per kept candidate per target; about
+#     9 flops each -- 3 sub or fma, squares and sums, compares), every
+#     accepted pair adds m W (about 15 flops); bytes: positions read and rho
+#     written once in sorted order (the accepted-candidate masks it writes
+#     are scratch, not algorithmic).  The reference's search tests every
+#     candidate of the 27 (9) cells; cand_all reports that count beside.
 #   force pass: only the accepted pairs of fluid targets (the masks of the
 #     density pass say which), about 45 flops each (pressure gradient, Monaghan
 #     viscosity, compensated sums); bytes: x, v, rho, P/rho^2, type read and a
@@ -94,6 +96,7 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
     pairs = sp.step(g, steps, scheme=SCHEME, **p)  # returns after a stream sync
     t1 = time.perf_counter()
     dens_pairs, force_pairs = g.step_pairs()
+    tests = g.step_tests()
     force_ms, force_n = g.profile_read(sp._native.SPH_KERNEL_FORCE)
     dens_ms, dens_n = g.profile_read(sp._native.SPH_KERNEL_DENSITY)
     build_ms, build_n = g.profile_read(sp._native.SPH_KERNEL_BUILD)
@@ -107,7 +110,7 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
         elapsed=el, pairs=pairs, value=2.0 * pairs / el, n=n, tile=T,
         force_ms=force_ms / max(force_n, 1), density_ms=dens_ms / max(dens_n, 1),
         build_ms=build_ms / max(build_n, 1), force_pairs_per_pass=force_pairs / steps,
-        density_pairs_per_pass=dens_pairs / steps,
+        density_pairs_per_pass=dens_pairs / steps, tests_per_pass=tests / steps,
         cand_fluid=cand_fluid, cand_all=cand_all,
     )
 
@@ -321,7 +324,7 @@ def main():
 
     def roofline(kernel):
         if kernel == "density":
-            flops = (FLOPS_PER_CANDIDATE * res["cand_all"]
+            flops = (FLOPS_PER_CANDIDATE * res["tests_per_pass"]
                      + FLOPS_PER_DENSITY_PAIR * res["density_pairs_per_pass"])
             ms, bpp = res["density_ms"], DENSITY_BYTES_PER_PARTICLE[prec]
             name = "density pass (candidate test, pair masks, rho = sum m W)"
@@ -374,6 +377,8 @@ def main():
             "neighbour_radius": st.R,
             "pairs_per_step": res["pairs"] / args.steps,
             "tile_targets": res["tile"],
+            "distance_tests_per_density_pass": res["tests_per_pass"],
+            "reference_candidates_per_pass": res["cand_all"],
             "parallelism": "single GPU",
         },
         "roofline": roofline(dominant),

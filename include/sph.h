@@ -166,6 +166,13 @@ int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps,
  * density passes (all targets) and force passes (fluid targets only). */
 int sph_step_pairs(sph_system* s, int64_t* density_pairs, int64_t* force_pairs);
 
+/* Diagnostics (no counterpart in the reference): distance tests the density
+ * passes of the last sph_step call performed, summed over its steps -- one
+ * bounding-box test per candidate per tile of targets, and one pair test per
+ * candidate the box test kept, per target of the tile.  The reference tests
+ * every candidate of the cells key + key_diff (sph_candidates). */
+int sph_step_tests(sph_system* s, int64_t* density_tests);
+
 /* Cell geometry as the reference computes it (src/structs.jl:63-82):
  * key_phase[3], key_lim[3], key_max, key_diff[27] (9 used in 2D). */
 int sph_key_geometry(sph_system* s, int64_t* key_phase, int64_t* key_lim,

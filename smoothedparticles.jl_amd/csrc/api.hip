@@ -441,12 +441,14 @@ int pair_grid(long long ntiles)
 }
 
 int run_pair(sph_system* s, int op, int self, int overwrite, unsigned long long* counter,
-             long long ntiles, int prof_kernel, int mask_mode, int only_flagged = 0)
+             long long ntiles, int prof_kernel, int mask_mode, int only_flagged = 0,
+             unsigned long long* tests = nullptr)
 {
     PairCtx c = make_ctx(s);
     c.self = self;
     c.overwrite = overwrite;
     c.pair_counter = counter;
+    c.test_counter = tests;
     c.mask_mode = s->use_masks ? mask_mode : MASK_TEST;
     c.only_flagged = only_flagged;
     hipEvent_t ev[2];
@@ -677,19 +679,19 @@ int run_steps(sph_system* s, int scheme, int nsteps)
     const int eos = scheme == SPH_SCHEME_WCSPH_SE ? SPH_OP_FIND_PRESSURE : SPH_OP_FIND_PRESSURE_RHO0;
     int rc;
     if (nsys(s) == 0) {
-        s->step_pairs[0] = s->step_pairs[1] = 0;
+        s->step_pairs[0] = s->step_pairs[1] = s->step_pairs[2] = 0;
         return 0;
     }
     // Tiles are taken from a work queue: the grid needs no tile count.
     const long long tiles_bound = s->tile_cap;
-    HIPC(hipMemsetAsync(s->pair_counter, 0, 2 * sizeof(unsigned long long), s->stream));
+    HIPC(hipMemsetAsync(s->pair_counter, 0, 3 * sizeof(unsigned long long), s->stream));
     for (int k = 0; k < nsteps; ++k) {
         hipEvent_t ev[2];
         prof_begin(s, ev);
         enqueue_build(s);
         prof_end(s, SPH_KERNEL_BUILD, ev);
         if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
-                           MASK_TEST_STORE)))
+                           MASK_TEST_STORE, 0, s->pair_counter + 2)))
             return rc;
         prof_begin(s, ev);
         enqueue_unary(s, eos, s->prho2);
@@ -702,7 +704,7 @@ int run_steps(sph_system* s, int scheme, int nsteps)
         prof_end(s, SPH_KERNEL_UNARY, ev);
         HIPC(hipGetLastError());
     }
-    HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 2 * sizeof(unsigned long long),
+    HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 3 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s->stream));
     if ((rc = sync_meta(s))) return rc;
     if (nsteps > 0) {
@@ -1123,6 +1125,13 @@ int sph_step_pairs(sph_system* s, int64_t* density_pairs, int64_t* force_pairs)
     if (!s) return fail(SPH_EINVAL, "null system");
     if (density_pairs) *density_pairs = (int64_t)s->step_pairs[0];
     if (force_pairs) *force_pairs = (int64_t)s->step_pairs[1];
+    return 0;
+}
+
+int sph_step_tests(sph_system* s, int64_t* density_tests)
+{
+    if (!s) return fail(SPH_EINVAL, "null system");
+    if (density_tests) *density_tests = (int64_t)s->step_pairs[2];
     return 0;
 }
 

@@ -281,8 +281,13 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
     const int c = tmp_key[u];
     const int s = cell_start[c], e = cell_start[c + 1];
     const int r = tmp_ref[u];
+    // ties (particles of different retired systems in the dead bucket) by
+    // scatter position, so the ranks are a permutation of the bucket
     int rank = 0;
-    for (int v = s; v < e; ++v) rank += tmp_ref[v] > r;
+    for (int v = s; v < e; ++v) {
+        const int rv = tmp_ref[v];
+        rank += rv > r || (rv == r && v < u);
+    }
     perm[s + rank] = tmp_idx[u];
 }
 

@@ -74,6 +74,12 @@ __device__ inline void count_pairs(const PairCtx& c, unsigned n)
     if (__lane_id() == 0 && n) atomicAdd(c.pair_counter, (unsigned long long)n);
 }
 
+// The wave's distance tests (the same value in every lane).
+__device__ inline void count_tests(const PairCtx& c, unsigned long long n)
+{
+    if (c.test_counter && __lane_id() == 0 && n) atomicAdd(c.test_counter, n);
+}
+
 // Compensated (2Sum) addition of a chunk partial into (s, e).
 __device__ inline void two_sum_add(float& s, float& e, float x)
 {
@@ -579,6 +585,7 @@ __device__ inline double lds_z(const Lds64Full& s, int j) { return s.zr[j].x; }
 template <int OUT>
 struct Density64 {
     static constexpr bool INTERACT = true;
+    static constexpr int WAVES = 5;  // occupancy the registers are fitted to
     using Lds = Lds64Pos;
     struct Tgt {
         double x, y, z;
@@ -635,6 +642,7 @@ struct Density64 {
 //   laminar   a += 2 kdW mu / rho0^2 u^3 v_pq
 struct Force64 {
     static constexpr bool INTERACT = true;
+    static constexpr int WAVES = 3;  // occupancy the registers are fitted to
     using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, vx, vy, vz, pr, rho;
@@ -721,6 +729,7 @@ struct Force64 {
 // internal_force! of tests/test_collision_2d.jl:75-78 (P/rho0^2, all targets).
 struct ForceRho064 {
     static constexpr bool INTERACT = true;
+    static constexpr int WAVES = 4;  // occupancy the registers are fitted to
     using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, P;
@@ -781,6 +790,7 @@ struct ForceRho064 {
 //   Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q))
 struct Continuity64 {
     static constexpr bool INTERACT = true;
+    static constexpr int WAVES = 4;  // occupancy the registers are fitted to
     using Lds = Lds64Full;
     struct Tgt {
         double x, y, z, vx, vy, vz, rho;
@@ -840,6 +850,7 @@ struct Continuity64 {
 template <bool EMIT>
 struct Pairs64 {
     static constexpr bool INTERACT = EMIT;
+    static constexpr int WAVES = 5;  // occupancy the registers are fitted to
     using Lds = Lds64Pos;
     struct Tgt {
         double x, y, z;
@@ -884,7 +895,8 @@ struct Pairs64 {
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
                               typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
-                              Bal<typename Op::Part>& bal, unsigned& npairs)
+                              Bal<typename Op::Part>& bal, unsigned& npairs,
+                              unsigned long long& ntests)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -978,7 +990,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
                 if (Op::INTERACT && cnt > 0) {
                     unsigned long long w = drop_low_bits(bal.w[tt], r0);
                     for (int k = 0; k < cnt; ++k) {
-                        const int b = __ffsll((long long)w) - 1;
+                        const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
                         w &= w - 1;
                         const int jj = rb + slot_of_bit<T>(b);
                         const double2 xy = lds.xy[jj];
@@ -999,11 +1011,14 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             __syncthreads();
         }
     }
+    // distance tests: the box test of every candidate, the pair test of
+    // every kept candidate for every target
+    if (!load_masks) ntests += (unsigned long long)total + (unsigned long long)fill * nvalid;
     Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
-__global__ __launch_bounds__(64) void k_pair64(PairCtx c)
+__global__ __launch_bounds__(64, Op::WAVES) void k_pair64(PairCtx c)
 {
     __shared__ typename Op::Lds lds;
     __shared__ Cands<ROWS, PERIODIC> L;
@@ -1016,9 +1031,12 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
     tile_cursor_init(tc);
     // accepted pairs of all the wave's tiles, one atomic per wave
     unsigned npairs = 0;
+    unsigned long long ntests = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
-        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L, bal, npairs);
+        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L, bal, npairs,
+                                            ntests);
     if (c.pair_counter) count_pairs(c, npairs);
+    count_tests(c, ntests);
 }
 
 // =============================================================== mixed ====
@@ -1026,6 +1044,12 @@ __global__ __launch_bounds__(64) void k_pair64(PairCtx c)
 // position of the tile's first target), converted while staging.  Per-lane
 // partial sums are plain FP32 within a 64-candidate chunk and folded into a
 // compensated (sum, error) pair after every chunk.
+
+// The hardware square root (v_sqrt_f32, within 1 ulp) instead of the
+// correctly rounded sequence the compiler emits for sqrtf (about 15
+// instructions).  The pair values need FP32 accuracy, not correct rounding:
+// which pairs interact is decided by the predicate, not by r.
+__device__ inline float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // LDS ring of kept candidates (FP32): (x, y, z, |x|^2) -- (x, y, z,
 // P/rho^2) for the force pass -- and, for the force pass, (vx, vy, vz, rho).
@@ -1085,7 +1109,7 @@ struct Density32 {
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
                                     int, int, int, float, float, float, float ss, Part& q)
     {
-        const float r = __builtin_sqrtf(ss);
+        const float r = sqrt_hw(ss);
         const float x = r * (float)d.g.inv_h;
         const float u = 1.0f - x;
         const float u2 = u * u;
@@ -1159,7 +1183,7 @@ struct Force32 {
                                     const Lds& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
-        const float r = __builtin_sqrtf(ss);
+        const float r = sqrt_hw(ss);
         const float u = 1.0f - r * (float)d.g.inv_h;
         const float u3 = u * u * u;
         const float4 v = s.v[jj];
@@ -1265,7 +1289,7 @@ template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
                               bool store_masks, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
                               Bal<typename Op::Part>& bal, typename Op::Tgt* tgts,
-                              unsigned& npairs)
+                              unsigned& npairs, unsigned long long& ntests)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -1398,7 +1422,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                     unsigned long long w = drop_low_bits(bal.w[tt], r0);
                     const int hp = first + tt;
                     for (int k = 0; k < cnt; ++k) {
-                        const int b = __ffsll((long long)w) - 1;
+                        const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
                         w &= w - 1;
                         const int jj = rb + slot_of_bit<T>(b);
                         const float4 a = lds.pos[jj];
@@ -1420,6 +1444,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             __syncthreads();
         }
     }
+    // distance tests: the box test of every candidate, the pair test of
+    // every kept candidate for every target
+    if (!load_masks) ntests += (unsigned long long)total + (unsigned long long)fill * nvalid;
     Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
@@ -1438,10 +1465,12 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair32(PairCtx c, int out_col
     tile_cursor_init(tc);
     // accepted pairs of all the wave's tiles, one atomic per wave
     unsigned npairs = 0;
+    unsigned long long ntests = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
         tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L, bal,
-                                            tgts, npairs);
+                                            tgts, npairs, ntests);
     if (c.pair_counter) count_pairs(c, npairs);
+    count_tests(c, ntests);
 }
 
 // ========================================================= diagnostics ====

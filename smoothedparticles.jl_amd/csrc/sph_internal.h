@@ -109,6 +109,7 @@ struct PairCtx {
     const int* refidx;
     const double* prho2;      // P/rho^2 per particle (force pass)
     unsigned long long* pair_counter;
+    unsigned long long* test_counter;  // distance tests of a testing pass (may be null)
     unsigned long long* emit_counter;
     long long* pair_out;      // pair emission (test hook)
     long long pair_cap;
@@ -311,7 +312,7 @@ struct sph_system {
     double* red_part;    // reduction partials
     long long red_part_n;
     double* red_out;     // 3 reduction results
-    unsigned long long step_pairs[2];  // pair counts of the last sph_step
+    unsigned long long step_pairs[3];  // density pairs, force pairs, density tests  // pair counts of the last sph_step
 
     // batch bookkeeping (host only)
     std::vector<int> slot_of_id;       // stable system id -> slot (-1 once compacted out)

@@ -226,6 +226,12 @@ class ParticleSystem:
         check(lib().sph_step_pairs(self._h, ctypes.byref(d), ctypes.byref(f)))
         return d.value, f.value
 
+    def step_tests(self):
+        """Distance tests of the density passes of the last step() call."""
+        t = ctypes.c_int64()
+        check(lib().sph_step_tests(self._h, ctypes.byref(t)))
+        return t.value
+
     def candidates(self):
         """(candidates over all targets, over fluid targets) of one pass."""
         a = ctypes.c_int64()

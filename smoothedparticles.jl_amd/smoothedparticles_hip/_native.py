@@ -104,6 +104,7 @@ SIGNATURES = {
     "sph_neighbour_pairs": (_i, [_p, _pi64, _i64, _pi64]),
     "sph_step": (_i, [_p, _i, ctypes.POINTER(sph_params), _i, _pi64]),
     "sph_step_pairs": (_i, [_p, _pi64, _pi64]),
+    "sph_step_tests": (_i, [_p, _pi64]),
     "sph_key_geometry": (_i, [_p, _pi64, _pi64, _pi64, _pi64, ctypes.POINTER(_i)]),
     "sph_set_tile": (_i, [_p, _i]),
     "sph_autotune": (_i, [_p, ctypes.POINTER(sph_params), ctypes.POINTER(_i)]),

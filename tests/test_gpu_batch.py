@@ -98,6 +98,28 @@ def test_retire_and_join_between_steps():
     b.close()
 
 
+def test_retire_two_systems_at_once():
+    """Two systems retired before one build share the dead bucket (their
+    reference indices overlap); the survivor is unaffected."""
+    print("batch retire two", flush=True)
+    (db, _), (tg, _), (db2, _) = small_states()
+    b = sp.Batch("fp64")
+    b.set_tile(TILE)
+    a = b.add_state(db)
+    t = b.add_state(tg, periodic=True)
+    c = b.add_state(db2)
+    b.init_rho0()
+    b.step(5)
+    b.retire(a)
+    b.retire(c)
+    b.step(5)
+    assert b.size() == (1, len(tg.x))
+    ref_t = run_alone(tg, True, "fp64", 10)
+    for f in ("x", "v", "rho", "rho0"):
+        assert np.array_equal(b.get(t, f), ref_t[f]), f
+    b.close()
+
+
 def test_removal_order_per_system():
     """Particles outside a reference system's box are removed with the
     swap-with-end renumbering of src/core.jl:72-81, per system."""
