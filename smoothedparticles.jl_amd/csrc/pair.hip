@@ -1106,6 +1106,9 @@ struct Density32 {
     }
     static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
                                     int, int, int, float, float, float, float ss, Part& q)
     {
@@ -1179,6 +1182,13 @@ struct Force32 {
         a.lx = a.ly = a.lz = a.fx = a.fy = a.fz = 0.0f;
     }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
+    // variant bit 0: Monaghan viscosity (avk != 0); bit 1: laminar (mu != 0)
+    static constexpr int NVAR = 4;
+    static __device__ int variant(const SysDesc& d)
+    {
+        return (d.ph.avk != 0.0 ? 1 : 0) | (d.ph.mu != 0.0 ? 2 : 0);
+    }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
@@ -1189,7 +1199,7 @@ struct Force32 {
         const float4 v = s.v[jj];
         float A = t.pr + s.pos[jj].w;
         const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
-        if (d.ph.avk != 0.0) {
+        if (V & 1) {
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
             if (vx < 0.0f) {
                 const float rhobar = 0.5f * (t.rho + v.w);
@@ -1199,7 +1209,7 @@ struct Force32 {
         }
         const float f = u3 * A;
         q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
-        if (d.ph.mu != 0.0) {
+        if (V & 2) {
             q.lx = fmaf(u3, dvx, q.lx); q.ly = fmaf(u3, dvy, q.ly); q.lz = fmaf(u3, dvz, q.lz);
         }
     }
@@ -1271,6 +1281,9 @@ struct Pairs32 {
     }
     static __device__ void init(Acc&) {}
     static __device__ void zero(Part&) {}
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
                                     int, int p, int q, float, float, float, float, Part&)
     {
@@ -1284,6 +1297,27 @@ struct Pairs32 {
     static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, const Acc&,
                                   int, bool, int) {}
 };
+
+// A lane's run: cnt accepted slots of w (lowest first) against target ht.
+template <int V, bool FLAT, int T, class Op>
+__device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
+                              const typename Op::Lds& lds, const int* q, unsigned long long w,
+                              int cnt, int rb, int hp, typename Op::Part& part)
+{
+    if constexpr (V < Op::NVAR) {
+#pragma unroll 2
+        for (int k = 0; k < cnt; ++k) {
+            const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
+            w &= w - 1;
+            const int jj = rb + slot_of_bit<T>(b);
+            const float4 a = lds.pos[jj];
+            const float dx = ht.x - a.x, dy = ht.y - a.y;
+            const float dz = FLAT ? 0.0f : ht.z - a.z;
+            const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            Op::template interact<V>(c, d, ht, lds, jj, hp, q[jj], dx, dy, dz, ss, part);
+        }
+    }
+}
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
@@ -1419,18 +1453,14 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 Op::zero(part);
                 if (Op::INTERACT && cnt > 0) {
                     const typename Op::Tgt ht = tgts[tt];
-                    unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                    const unsigned long long w = drop_low_bits(bal.w[tt], r0);
                     const int hp = first + tt;
-                    for (int k = 0; k < cnt; ++k) {
-                        const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
-                        w &= w - 1;
-                        const int jj = rb + slot_of_bit<T>(b);
-                        const float4 a = lds.pos[jj];
-                        const float dx = ht.x - a.x, dy = ht.y - a.y;
-                        const float dz = FLAT ? 0.0f : ht.z - a.z;
-                        const float ss =
-                            FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        Op::interact(c, d, ht, lds, jj, hp, L.q[jj], dx, dy, dz, ss, part);
+                    // the operator's uniform switches are taken outside the loop
+                    switch (Op::variant(d)) {
+                    case 0: walk32<0, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    case 1: walk32<1, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    case 2: walk32<2, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    default: walk32<3, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
                     }
                 }
                 bal.part[lane] = part;
