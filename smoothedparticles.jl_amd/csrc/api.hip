@@ -184,7 +184,8 @@ void free_particle_arrays(sph_system* s)
     }
     dfree(s, s->slot, cap); dfree(s, s->tmp_idx, cap); dfree(s, s->tmp_key, cap);
     dfree(s, s->tmp_ref, cap); dfree(s, s->perm, cap); dfree(s, s->inv, cap);
-    dfree(s, s->tiles, s->tile_cap); dfree(s, s->prho2, cap); dfree(s, s->stage, 9 * cap);
+    dfree(s, s->tiles, s->tile_cap); dfree(s, s->kept_n, s->tile_cap);
+    dfree(s, s->prho2, cap); dfree(s, s->stage, 9 * cap);
     dfree(s, s->red_part, s->red_part_n);
 }
 
@@ -235,7 +236,8 @@ int ensure_capacity(sph_system* s, long long need)
     if ((rc = dalloc(s, &s->slot, ncap)) || (rc = dalloc(s, &s->tmp_idx, ncap)) ||
         (rc = dalloc(s, &s->tmp_key, ncap)) || (rc = dalloc(s, &s->tmp_ref, ncap)) ||
         (rc = dalloc(s, &s->perm, ncap)) || (rc = dalloc(s, &s->inv, ncap)) ||
-        (rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->prho2, ncap)) ||
+        (rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->kept_n, s->tile_cap)) ||
+        (rc = dalloc(s, &s->prho2, ncap)) ||
         (rc = dalloc(s, &s->stage, 9 * ncap)) || (rc = dalloc(s, &s->red_part, s->red_part_n)))
         return rc;
     return 0;
@@ -271,8 +273,10 @@ int ensure_cells(sph_system* s, long long buckets)
     }
     if (s->cap > 0) {  // the tile bound grows with the cell count
         dfree(s, s->tiles, s->tile_cap);
+        dfree(s, s->kept_n, s->tile_cap);
         s->tile_cap = tile_bound(s);
-        if ((rc = dalloc(s, &s->tiles, s->tile_cap))) return rc;
+        if ((rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->kept_n, s->tile_cap)))
+            return rc;
     }
     for (int i = 0; i < nsys(s); ++i)
         enqueue_fill_int(s, s->cell_sys + s->sys_host[i].cell_base, i, s->sys_host[i].g.key_max);
@@ -402,6 +406,8 @@ PairCtx make_ctx(sph_system* s)
     c.refidx = s->icols[s->cur] + I_REF * s->cap;
     c.prho2 = s->prho2;
     c.masks = s->masks;
+    c.kept = s->kept;
+    c.kept_n = s->kept_n;
     c.mask_mode = MASK_TEST;
     return c;
 }
@@ -426,8 +432,21 @@ int ensure_masks(sph_system* s)
         }
         s->mask_cap = cap;
     }
-    const int ok = 1;
-    HIPC(hipMemcpyAsync(s->meta + META_MASK_OK, &ok, sizeof(int), hipMemcpyHostToDevice, s->stream));
+    // kept-candidate lists: 64 entries per chunk of a tile (T mask words)
+    const long long kneed = (long long)(need / s->tile_T * 64);
+    if (kneed > s->kept_cap) {
+        dfree(s, s->kept, s->kept_cap);
+        s->kept_cap = 0;
+        const long long cap = kneed + kneed / 4 + 4096;
+        if (dalloc(s, &s->kept, cap)) {
+            s->kept = nullptr;  // no lists: the loading passes stream and cull
+            g_err.clear();
+        } else {
+            s->kept_cap = cap;
+        }
+    }
+    const int ok[2] = {1, s->kept ? 1 : 0};
+    HIPC(hipMemcpyAsync(s->meta + META_MASK_OK, ok, sizeof(ok), hipMemcpyHostToDevice, s->stream));
     HIPC(hipStreamSynchronize(s->stream));
     return 0;
 }
@@ -538,6 +557,7 @@ int destroy(sph_system* s)
     dfree(s, s->meta, META_SLOTS); dfree(s, s->pair_counter, 4); dfree(s, s->red_out, 4);
     dfree(s, s->queue, NQUEUE * QSTRIDE);
     dfree(s, s->masks, s->mask_cap); dfree(s, s->mask_total, 1);
+    dfree(s, s->kept, s->kept_cap);
     dfree(s, s->sysd, s->sysd_cap); dfree(s, s->sys_cnt, SYSCNT * s->sys_cnt_cap);
     if (s->meta_host) (void)hipHostFree(s->meta_host);
     if (s->sys_cnt_host) (void)hipHostFree(s->sys_cnt_host);

@@ -385,13 +385,15 @@ __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__
 }
 
 // Tile table: tile k of cell c covers targets cell_start[c] + k*T ...; its
-// pair masks start at mask_start[c] + k * chunks[c] * T.
+// pair masks start at mask_start[c] + k * chunks[c] * T, its kept-candidate
+// list at 64 / T times that.
 __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_start,
                                                const int* __restrict__ tile_start,
                                                const int* __restrict__ chunks,
                                                const unsigned long long* __restrict__ mask_start,
-                                               int ncells, int T, long long mask_cap, int4* tiles,
-                                               int* meta, unsigned long long* mask_total)
+                                               int ncells, int T, long long mask_cap,
+                                               long long kept_cap, int4* tiles, int* meta,
+                                               unsigned long long* mask_total)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c == 0) {
@@ -399,6 +401,8 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
         const unsigned long long need = mask_start[ncells];
         *mask_total = need;
         meta[META_MASK_OK] = (long long)need <= mask_cap ? 1 : 0;
+        // 64 kept-candidate entries per chunk of a tile, T mask words
+        meta[META_KEPT_OK] = (long long)(need / T * 64) <= kept_cap ? 1 : 0;
     }
     if (c >= ncells) return;
     const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
@@ -430,7 +434,8 @@ void enqueue_tiles(sph_system* s)
          s->mask_start, s->scan_part64);
     k_tiles<<<nbc, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
                                         s->mask_start, ncells, s->tile_T,
-                                        s->masks ? s->mask_cap : 0, s->tiles, s->meta,
+                                        s->masks ? s->mask_cap : 0,
+                                        s->kept ? s->kept_cap : 0, s->tiles, s->meta,
                                         s->mask_total);
 }
 

@@ -262,7 +262,8 @@ __device__ inline int lane_rank(unsigned long long m)
 
 // Append the lanes' kept candidates to the ring (list order).  fill: kept so
 // far; nself: occurrences of the first target so far.  Returns the lane's
-// ring slot, -1 if its candidate is not kept.
+// kept position (its ring slot is pos & 127), -1 if its candidate is not
+// kept.
 template <int ROWS, bool PER>
 __device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool keep, int q, int rg,
                               int first)
@@ -270,16 +271,15 @@ __device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool k
     const unsigned long long km = __ballot(keep);
     const unsigned long long sm = __ballot(keep && q == first);
     const int pos = fill + lane_rank(km);
-    int r = -1;
     if (keep) {
-        r = pos & 127;
+        const int r = pos & 127;
         L.q[r] = q;
         if constexpr (PER) L.rg[r] = rg;
         if (q == first) L.self0[nself + lane_rank(sm)] = pos;
     }
     fill += __popcll(km);
     nself += __popcll(sm);
-    return r;
+    return keep ? pos : -1;
 }
 
 // Clear the bits of the chunk's slots (kept positions head ... head + 63)
@@ -894,6 +894,7 @@ struct Pairs64 {
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
+                              bool kept_ok,
                               typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
                               Bal<typename Op::Part>& bal, unsigned& npairs,
                               unsigned long long& ntests)
@@ -915,25 +916,55 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double s_thr = d.g.s_thr;
-    const Box<double> box = tile_box<double, FLAT>(valid, tg.x, tg.y, tg.z);
-    int total;
-    const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    // a pass that loads the masks takes the kept candidates from the list the
+    // testing pass wrote (reference systems); otherwise it streams the
+    // candidate list and culls it again
+    const bool use_kept = !PERIODIC && load_masks && kept_ok;
+    const bool store_kept = !PERIODIC && store_masks && kept_ok;
+    const long long lbase = mbase / T * 64;
+    const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
+    int total = 0, nr = 0;
+    Box<double> box{};
+    if (!use_kept) {
+        box = tile_box<double, FLAT>(valid, tg.x, tg.y, tg.z);
+        nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    }
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
     int cur = 0, fill = 0, head = 0, nself = 0, chunk = 0;
-    for (int base = 0; base < total; base += 64) {
-        Raw raw;
-        int rq = 0, rrg = 0;
-        cur = advance_range(L, nr, cur, base);
-        fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
-        // keep the candidates within reach of the box, and the tile's targets
-        const bool keep = base + lane < total &&
-                          (box_dist2<FLAT>(box, raw.x, raw.y, raw.z) <= s_thr ||
-                           (unsigned)(rq - first) < (unsigned)nvalid);
-        const int r = compact(L, fill, nself, keep, rq, rrg, first);
-        if (r >= 0) Op::put(raw, r, lds);
-        const bool last = base + 64 >= total;
+    const int nstep = use_kept ? (nk + 63) / 64 : (total + 63) / 64;
+    unsigned long long kword = 0;  // the chunk's mask word, loaded with its list entries
+    for (int step = 0; step < nstep; ++step) {
+        bool last;
+        if (use_kept) {
+            // one chunk of the kept list per step
+            if (s == 0 && act) kword = c.masks[mbase + (long long)step * T + t];
+            const int k = head + lane;
+            if (k < nk) {
+                const int q = c.kept[lbase + k];
+                Op::put(Op::template fetch<false>(c, q, Seg{}), k & 127, lds);
+                L.q[k & 127] = q;
+            }
+            fill = min(nk, head + 64);
+            last = true;
+        } else {
+            const int base = step * 64;
+            Raw raw;
+            int rq = 0, rrg = 0;
+            cur = advance_range(L, nr, cur, base);
+            fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
+            // keep the candidates within reach of the box, and the tile's targets
+            const bool keep = base + lane < total &&
+                              (box_dist2<FLAT>(box, raw.x, raw.y, raw.z) <= s_thr ||
+                               (unsigned)(rq - first) < (unsigned)nvalid);
+            const int pos = compact(L, fill, nself, keep, rq, rrg, first);
+            if (pos >= 0) {
+                Op::put(raw, pos & 127, lds);
+                if (store_kept) c.kept[lbase + pos] = rq;
+            }
+            last = base + 64 >= total;
+        }
         __syncthreads();
         while (fill - head >= 64 || (last && fill > head)) {
             if (fill - head < 64) {
@@ -948,7 +979,9 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             const int rb = head & 64;  // ring half of the chunk
             // accepted-slot word of target t (owner lanes publish it)
             unsigned long long word = 0;
-            if (load_masks) {
+            if (use_kept) {
+                word = kword;
+            } else if (load_masks) {
                 if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
             } else {
                 // 32-bit words, slots shifted in from the highest bit down
@@ -1014,6 +1047,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     // distance tests: the box test of every candidate, the pair test of
     // every kept candidate for every target
     if (!load_masks) ntests += (unsigned long long)total + (unsigned long long)fill * nvalid;
+    if (store_kept && lane == 0) c.kept_n[tile] = fill;
     Op::finish(c, d, p, tg, acc, T, act && s == 0);
 }
 
@@ -1027,14 +1061,15 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair64(PairCtx c)
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
+    const bool kept_ok = ok && c.kept != nullptr && ld_const(c.meta + META_KEPT_OK) != 0;
     TileCursor tc;
     tile_cursor_init(tc);
     // accepted pairs of all the wave's tiles, one atomic per wave
     unsigned npairs = 0;
     unsigned long long ntests = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
-        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, lds, L, bal, npairs,
-                                            ntests);
+        tile64<ROWS, FLAT, PERIODIC, T, Op>(c, tile, load_masks, store_masks, kept_ok, lds, L, bal,
+                                            npairs, ntests);
     if (c.pair_counter) count_pairs(c, npairs);
     count_tests(c, ntests);
 }
@@ -1321,7 +1356,7 @@ __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
+                              bool store_masks, bool kept_ok, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
                               Bal<typename Op::Part>& bal, typename Op::Tgt* tgts,
                               unsigned& npairs, unsigned long long& ntests)
 {
@@ -1354,30 +1389,61 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const float thr_in = (float)(s_thr * (1.0 - BAND)) - p2;
     const float thr_out = (float)(s_thr * (1.0 + BAND)) - p2;
     const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
-    // the cull keeps everything within the guard band of the box
-    const Box<float> box = tile_box<float, FLAT>(valid, tg.x, tg.y, tz);
-    const float cull_thr = (float)(s_thr * (1.0 + BAND));
     // the targets' records, read by the lanes that walk their pairs
     if (s == 0) tgts[t] = tg;
-    int total;
-    const int nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    // a pass that loads the masks takes the kept candidates from the list the
+    // testing pass wrote (reference systems); otherwise it streams the
+    // candidate list and culls it again, keeping everything within the guard
+    // band of the box
+    const bool use_kept = !PERIODIC && load_masks && kept_ok;
+    const bool store_kept = !PERIODIC && store_masks && kept_ok;
+    const long long lbase = mbase / T * 64;
+    const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
+    const float cull_thr = (float)(s_thr * (1.0 + BAND));
+    int total = 0, nr = 0;
+    Box<float> box{};
+    if (!use_kept) {
+        box = tile_box<float, FLAT>(valid, tg.x, tg.y, tz);
+        nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
+    }
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
     int cur = 0, fill = 0, head = 0, nself = 0, chunk = 0;
-    for (int base = 0; base < total; base += 64) {
-        Raw raw;
-        int rq = 0, rrg = 0;
-        cur = advance_range(L, nr, cur, base);
-        fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
-        // keep the candidates within reach of the box, and the tile's targets
-        const float4 a = rel_pos32<FLAT>(raw, ox, oy, oz);
-        const bool keep = base + lane < total &&
-                          (box_dist2<FLAT>(box, a.x, a.y, a.z) <= cull_thr ||
-                           (unsigned)(rq - first) < (unsigned)nvalid);
-        const int r = compact(L, fill, nself, keep, rq, rrg, first);
-        if (r >= 0) Op::template put<FLAT>(raw, r, ox, oy, oz, lds);
-        const bool last = base + 64 >= total;
+    const int nstep = use_kept ? (nk + 63) / 64 : (total + 63) / 64;
+    unsigned long long kword = 0;  // the chunk's mask word, loaded with its list entries
+    for (int step = 0; step < nstep; ++step) {
+        bool last;
+        if (use_kept) {
+            // one chunk of the kept list per step
+            if (s == 0 && act) kword = c.masks[mbase + (long long)step * T + t];
+            const int k = head + lane;
+            if (k < nk) {
+                const int q = c.kept[lbase + k];
+                Op::template put<FLAT>(Op::template fetch<false>(c, q, Seg{}), k & 127, ox, oy,
+                                       oz, lds);
+                L.q[k & 127] = q;
+            }
+            fill = min(nk, head + 64);
+            last = true;
+        } else {
+            const int base = step * 64;
+            Raw raw;
+            int rq = 0, rrg = 0;
+            cur = advance_range(L, nr, cur, base);
+            fetch_slot<ROWS, PERIODIC, Op>(c, L, total, cur, base + lane, raw, rq, rrg);
+            // keep the candidates within reach of the box, and the tile's targets
+            const float4 a = rel_pos32<FLAT>(raw, ox, oy, oz);
+            const bool keep = base + lane < total &&
+                              (box_dist2<FLAT>(box, a.x, a.y, a.z) <= cull_thr ||
+                               (unsigned)(rq - first) < (unsigned)nvalid);
+            const int pos = compact(L, fill, nself, keep, rq, rrg, first);
+            if (pos >= 0) {
+                Op::template put<FLAT>(raw, pos & 127, ox, oy, oz, lds);
+                if (store_kept) c.kept[lbase + pos] = rq;
+            }
+            last = base + 64 >= total;
+        }
         __syncthreads();
         while (fill - head >= 64 || (last && fill > head)) {
             if (fill - head < 64) {
@@ -1393,7 +1459,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             const int rb = head & 64;  // ring half of the chunk
             // accepted-slot word of target t (owner lanes publish it)
             unsigned long long word = 0;
-            if (load_masks) {
+            if (use_kept) {
+                word = kword;
+            } else if (load_masks) {
                 if (s == 0 && act) word = c.masks[mbase + (long long)chunk * T + t];
             } else {
                 // 32-bit words, slots shifted in from the highest bit down
@@ -1477,6 +1545,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     // distance tests: the box test of every candidate, the pair test of
     // every kept candidate for every target
     if (!load_masks) ntests += (unsigned long long)total + (unsigned long long)fill * nvalid;
+    if (store_kept && lane == 0) c.kept_n[tile] = fill;
     Op::finish(c, d, p, tg, acc, T, act && s == 0, out_col);
 }
 
@@ -1491,14 +1560,15 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair32(PairCtx c, int out_col
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
     const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
+    const bool kept_ok = ok && c.kept != nullptr && ld_const(c.meta + META_KEPT_OK) != 0;
     TileCursor tc;
     tile_cursor_init(tc);
     // accepted pairs of all the wave's tiles, one atomic per wave
     unsigned npairs = 0;
     unsigned long long ntests = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
-        tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, lds, L, bal,
-                                            tgts, npairs, ntests);
+        tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, kept_ok, lds,
+                                            L, bal, tgts, npairs, ntests);
     if (c.pair_counter) count_pairs(c, npairs);
     count_tests(c, ntests);
 }

@@ -43,6 +43,7 @@ enum Meta {
     META_NCELLS = 4,  // non-empty cells
     META_QUEUE = 5,   // (unused; the pair kernels' tile queues are separate)
     META_MASK_OK = 6, // the pair-mask buffer holds this cell list's masks
+    META_KEPT_OK = 7, // the kept-candidate buffer holds this cell list's lists
     META_SLOTS = 8
 };
 
@@ -101,6 +102,8 @@ struct PairCtx {
     int cap;
     const int4* tiles;        // {global cell, first target, mask base lo, mask base hi}
     unsigned long long* masks;
+    int* kept;                // kept candidates of each tile (64 per mask chunk)
+    int* kept_n;              // kept candidates per tile
     int mask_mode;
     const int* cell_start;
     const int* meta;
@@ -303,6 +306,9 @@ struct sph_system {
     unsigned long long* mask_start;  // cells + 1: first mask word of each cell
     unsigned long long* scan_part64;
     unsigned long long* masks;       // pair masks of the current cell list
+    int* kept;                       // kept-candidate lists of the tiles (64 per chunk)
+    long long kept_cap;              // entries allocated
+    int* kept_n;                     // tile_cap: kept candidates per tile
     long long mask_cap;              // words allocated
     unsigned long long* mask_total;  // device: words needed (1 value)
     bool masks_valid;                // masks match the cell list and positions
