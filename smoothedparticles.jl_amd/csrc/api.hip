@@ -185,6 +185,7 @@ void free_particle_arrays(sph_system* s)
     dfree(s, s->slot, cap); dfree(s, s->tmp_idx, cap); dfree(s, s->tmp_key, cap);
     dfree(s, s->tmp_ref, cap); dfree(s, s->perm, cap); dfree(s, s->inv, cap);
     dfree(s, s->tiles, s->tile_cap); dfree(s, s->kept_n, s->tile_cap);
+    dfree(s, s->boxes, 6 * s->tile_cap);
     dfree(s, s->prho2, cap); dfree(s, s->stage, 9 * cap);
     dfree(s, s->red_part, s->red_part_n);
 }
@@ -237,7 +238,7 @@ int ensure_capacity(sph_system* s, long long need)
         (rc = dalloc(s, &s->tmp_key, ncap)) || (rc = dalloc(s, &s->tmp_ref, ncap)) ||
         (rc = dalloc(s, &s->perm, ncap)) || (rc = dalloc(s, &s->inv, ncap)) ||
         (rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->kept_n, s->tile_cap)) ||
-        (rc = dalloc(s, &s->prho2, ncap)) ||
+        (rc = dalloc(s, &s->boxes, 6 * s->tile_cap)) || (rc = dalloc(s, &s->prho2, ncap)) ||
         (rc = dalloc(s, &s->stage, 9 * ncap)) || (rc = dalloc(s, &s->red_part, s->red_part_n)))
         return rc;
     return 0;
@@ -274,8 +275,10 @@ int ensure_cells(sph_system* s, long long buckets)
     if (s->cap > 0) {  // the tile bound grows with the cell count
         dfree(s, s->tiles, s->tile_cap);
         dfree(s, s->kept_n, s->tile_cap);
+        dfree(s, s->boxes, 6 * s->tile_cap);
         s->tile_cap = tile_bound(s);
-        if ((rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->kept_n, s->tile_cap)))
+        if ((rc = dalloc(s, &s->tiles, s->tile_cap)) || (rc = dalloc(s, &s->kept_n, s->tile_cap)) ||
+            (rc = dalloc(s, &s->boxes, 6 * s->tile_cap)))
             return rc;
     }
     for (int i = 0; i < nsys(s); ++i)
@@ -408,6 +411,7 @@ PairCtx make_ctx(sph_system* s)
     c.masks = s->masks;
     c.kept = s->kept;
     c.kept_n = s->kept_n;
+    c.boxes = s->boxes;
     c.mask_mode = MASK_TEST;
     return c;
 }

@@ -416,6 +416,31 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
     }
 }
 
+// Bounding box of each tile's targets (absolute FP64), read by the pair
+// kernels to cull candidates.  NaN coordinates are ignored (fmin/fmax).
+template <bool FLAT>
+__global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ tiles,
+                                                    const int* __restrict__ cell_start,
+                                                    const int* meta, const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ z, int T,
+                                                    double* boxes)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= meta[META_NTILES]) return;
+    const int4 td = tiles[i];
+    const int first = td.y;
+    const int end = min(first + T, cell_start[td.x + 1]);
+    double b[6] = {1.0e30, -1.0e30, 1.0e30, -1.0e30, 0.0, 0.0};
+    if (!FLAT) b[4] = 1.0e30, b[5] = -1.0e30;
+    for (int p = first; p < end; ++p) {
+        b[0] = fmin(b[0], x[p]); b[1] = fmax(b[1], x[p]);
+        b[2] = fmin(b[2], y[p]); b[3] = fmax(b[3], y[p]);
+        if (!FLAT) { b[4] = fmin(b[4], z[p]); b[5] = fmax(b[5], z[p]); }
+    }
+    for (int k = 0; k < 6; ++k) boxes[6 * (long long)i + k] = b[k];
+}
+
 // ------------------------------------------------------------ host side --
 static inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -437,6 +462,12 @@ void enqueue_tiles(sph_system* s)
                                         s->masks ? s->mask_cap : 0,
                                         s->kept ? s->kept_cap : 0, s->tiles, s->meta,
                                         s->mask_total);
+    const double* c = s->cols[s->cur];
+    const long long cap = s->cap;
+    const unsigned nbt = nblocks(s->tile_cap, 256);
+    auto boxes = s->flat ? k_tile_boxes<true> : k_tile_boxes<false>;
+    boxes<<<nbt, 256, 0, s->stream>>>(s->tiles, s->cell_start, s->meta, c + C_X * cap,
+                                      c + C_Y * cap, c + C_Z * cap, s->tile_T, s->boxes);
 }
 
 void enqueue_build(sph_system* s)

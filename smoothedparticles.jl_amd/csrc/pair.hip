@@ -297,57 +297,13 @@ __device__ inline void drop_self(const Cands<ROWS, PER>& L, int nself, int head,
     }
 }
 
-// Bounding box of the tile's valid targets (the same value in every lane).
+// Bounding box of a tile's targets, computed with the tile table in absolute
+// FP64 coordinates; the mixed kernels convert it to FP32 relative to the
+// tile's first target.
 template <class V>
 struct Box {
     V x0, x1, y0, y1, z0, z1;
 };
-
-__device__ inline float uniform(float v)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-__device__ inline double uniform(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
-    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-template <class V>
-__device__ inline V wave_minv(V v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-    return uniform(v);
-}
-
-template <class V>
-__device__ inline V wave_maxv(V v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return uniform(v);
-}
-
-template <class V, bool FLAT>
-__device__ inline Box<V> tile_box(bool valid, V x, V y, V z)
-{
-    const V big = V(1.0e30);
-    Box<V> b;
-    b.x0 = wave_minv(valid ? x : big);
-    b.x1 = wave_maxv(valid ? x : -big);
-    b.y0 = wave_minv(valid ? y : big);
-    b.y1 = wave_maxv(valid ? y : -big);
-    b.z0 = b.z1 = V(0);
-    if (!FLAT) {
-        b.z0 = wave_minv(valid ? z : big);
-        b.z1 = wave_maxv(valid ? z : -big);
-    }
-    return b;
-}
 
 // Squared distance of (x, y, z) to the box (0 inside), evaluated in the
 // predicate's operation order.
@@ -473,6 +429,43 @@ __device__ inline V shfl_value(const V& v, int src)
     return r;
 }
 
+// Wave sum of v, 0 <= v < 128, from bit-sliced ballots: no cross-lane
+// data movement through LDS, the result is uniform.
+__device__ inline int wave_sum7(int v)
+{
+    int sum = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) sum += __popcll(__ballot((v >> b) & 1)) << b;
+    return sum;
+}
+
+// Sum of v over the lanes below this one, 0 <= v < 128.
+__device__ inline int lanes_below_sum7(int v)
+{
+    int sum = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) sum += lane_rank(__ballot((v >> b) & 1)) << b;
+    return sum;
+}
+
+// ceil(65536 / d) for a uniform 1 <= d <= 64: a reciprocal estimate, then
+// corrected to the exact value.
+__device__ inline int run_magic(int d)
+{
+    int m = (int)(65536.0f * __builtin_amdgcn_rcpf((float)d)) + 1;
+    while ((m - 1) * d >= 65536) --m;
+    while (m * d < 65536) ++m;
+    return m;
+}
+
+// ceil(pop / B) for 0 <= pop <= 64: with m = ceil(65536 / B) and B <= 64,
+// floor(n m / 65536) = floor(n / B) for n < 1024.
+__device__ inline int runs_of(int pop, int B, int m)
+{
+    if (pop <= 0) return 0;
+    return B <= 64 ? ((pop + B - 1) * m) >> 16 : (pop + B - 1) / B;
+}
+
 // Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
 // only, 0 elsewhere).  Returns the total; sets the lane's run (target tt,
 // first rank r0, count cnt) and, for owner lanes, the first run and the
@@ -481,28 +474,25 @@ template <class Part>
 __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int& r0, int& cnt,
                                  int& first_run, int& nruns)
 {
-    const int tot = wave_sum(pop);
+    const int tot = wave_sum7(pop);
     tt = -1;
     r0 = cnt = first_run = nruns = 0;
     if (tot == 0) return 0;
-    int B = (tot + 63) / 64;
-    int need = wave_sum(pop > 0 ? (pop + B - 1) / B : 0);
+    int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
+    int m = run_magic(B);
+    int need = wave_sum7(runs_of(pop, B, m));
     if (need > 64) {
         ++B;
-        need = wave_sum(pop > 0 ? (pop + B - 1) / B : 0);
+        m = B <= 64 ? run_magic(B) : 0;
+        need = wave_sum7(runs_of(pop, B, m));
         if (need > 64) {
             const int nz = __popcll(__ballot(pop > 0));
             B = nz < 64 ? (tot + (64 - nz) - 1) / (64 - nz) : wave_max(pop);
+            m = B <= 64 ? run_magic(B) : 0;
         }
     }
-    nruns = pop > 0 ? (pop + B - 1) / B : 0;
-    int incl = nruns;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d);
-        if (lane >= d) incl += v;
-    }
-    first_run = incl - nruns;
+    nruns = runs_of(pop, B, m);
+    first_run = lanes_below_sum7(nruns);
     bal.lane_t[lane] = -1;
     __syncthreads();
     for (int k = 0; k < nruns; ++k) bal.lane_t[first_run + k] = lane;
@@ -926,7 +916,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     int total = 0, nr = 0;
     Box<double> box{};
     if (!use_kept) {
-        box = tile_box<double, FLAT>(valid, tg.x, tg.y, tg.z);
+        box = ld_const(reinterpret_cast<const Box<double>*>(c.boxes) + tile);
         nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     }
     __syncthreads();
@@ -1403,7 +1393,12 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     int total = 0, nr = 0;
     Box<float> box{};
     if (!use_kept) {
-        box = tile_box<float, FLAT>(valid, tg.x, tg.y, tz);
+        // RN(x - ox) is monotonic in x: the box of the relative positions
+        // is the relative box
+        const Box<double> b = ld_const(reinterpret_cast<const Box<double>*>(c.boxes) + tile);
+        box.x0 = (float)(b.x0 - ox); box.x1 = (float)(b.x1 - ox);
+        box.y0 = (float)(b.y0 - oy); box.y1 = (float)(b.y1 - oy);
+        box.z0 = (float)(b.z0 - oz); box.z1 = (float)(b.z1 - oz);
         nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     }
     __syncthreads();

@@ -104,6 +104,7 @@ struct PairCtx {
     unsigned long long* masks;
     int* kept;                // kept candidates of each tile (64 per mask chunk)
     int* kept_n;              // kept candidates per tile
+    const double* boxes;      // bounding box of each tile's targets (6 doubles)
     int mask_mode;
     const int* cell_start;
     const int* meta;
@@ -309,6 +310,7 @@ struct sph_system {
     int* kept;                       // kept-candidate lists of the tiles (64 per chunk)
     long long kept_cap;              // entries allocated
     int* kept_n;                     // tile_cap: kept candidates per tile
+    double* boxes;                   // 6 * tile_cap: bounding boxes of the tiles' targets
     long long mask_cap;              // words allocated
     unsigned long long* mask_total;  // device: words needed (1 value)
     bool masks_valid;                // masks match the cell list and positions
