@@ -284,6 +284,7 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
     // ties (particles of different retired systems in the dead bucket) by
     // scatter position, so the ranks are a permutation of the bucket
     int rank = 0;
+#pragma unroll 8
     for (int v = s; v < e; ++v) {
         const int rv = tmp_ref[v];
         rank += rv > r || (rv == r && v < u);
@@ -417,7 +418,8 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
 }
 
 // Bounding box of each tile's targets (absolute FP64), read by the pair
-// kernels to cull candidates.  NaN coordinates are ignored (fmin/fmax).
+// kernels to cull candidates.  A wave covers 64 / T tiles at a time, one
+// lane per target; NaN coordinates are ignored (fmin/fmax).
 template <bool FLAT>
 __global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ tiles,
                                                     const int* __restrict__ cell_start,
@@ -426,19 +428,37 @@ __global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ til
                                                     const double* __restrict__ z, int T,
                                                     double* boxes)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= meta[META_NTILES]) return;
-    const int4 td = tiles[i];
-    const int first = td.y;
-    const int end = min(first + T, cell_start[td.x + 1]);
-    double b[6] = {1.0e30, -1.0e30, 1.0e30, -1.0e30, 0.0, 0.0};
-    if (!FLAT) b[4] = 1.0e30, b[5] = -1.0e30;
-    for (int p = first; p < end; ++p) {
-        b[0] = fmin(b[0], x[p]); b[1] = fmax(b[1], x[p]);
-        b[2] = fmin(b[2], y[p]); b[3] = fmax(b[3], y[p]);
-        if (!FLAT) { b[4] = fmin(b[4], z[p]); b[5] = fmax(b[5], z[p]); }
+    const int ntiles = meta[META_NTILES];
+    const int lane = threadIdx.x & 63;
+    const int t = lane & (T - 1);
+    const long long nwaves = (long long)gridDim.x * 4;
+    for (long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
+         wave * (64 / T) < ntiles; wave += nwaves) {
+        const long long i = wave * (64 / T) + lane / T;
+        double b[6] = {1.0e30, -1.0e30, 1.0e30, -1.0e30, 0.0, 0.0};
+        if (!FLAT) b[4] = 1.0e30, b[5] = -1.0e30;
+        if (i < ntiles) {
+            const int4 td = tiles[i];
+            const int p = td.y + t;
+            if (p < cell_start[td.x + 1]) {
+                b[0] = b[1] = x[p];
+                b[2] = b[3] = y[p];
+                if (!FLAT) b[4] = b[5] = z[p];
+            }
+        }
+        for (int o = 1; o < T; o <<= 1) {
+            b[0] = fmin(b[0], __shfl_xor(b[0], o)); b[1] = fmax(b[1], __shfl_xor(b[1], o));
+            b[2] = fmin(b[2], __shfl_xor(b[2], o)); b[3] = fmax(b[3], __shfl_xor(b[3], o));
+            if (!FLAT) {
+                b[4] = fmin(b[4], __shfl_xor(b[4], o)); b[5] = fmax(b[5], __shfl_xor(b[5], o));
+            }
+        }
+        if (i < ntiles && t < 6) {
+            double v = b[0];
+            for (int k = 1; k < 6; ++k) v = t == k ? b[k] : v;
+            boxes[6 * i + t] = v;
+        }
     }
-    for (int k = 0; k < 6; ++k) boxes[6 * (long long)i + k] = b[k];
 }
 
 // ------------------------------------------------------------ host side --
@@ -464,7 +484,7 @@ void enqueue_tiles(sph_system* s)
                                         s->mask_total);
     const double* c = s->cols[s->cur];
     const long long cap = s->cap;
-    const unsigned nbt = nblocks(s->tile_cap, 256);
+    const unsigned nbt = 2048;  // waves loop over the tiles
     auto boxes = s->flat ? k_tile_boxes<true> : k_tile_boxes<false>;
     boxes<<<nbt, 256, 0, s->stream>>>(s->tiles, s->cell_start, s->meta, c + C_X * cap,
                                       c + C_Y * cap, c + C_Z * cap, s->tile_T, s->boxes);
