@@ -410,6 +410,7 @@ PairCtx make_ctx(sph_system* s)
     c.prho2 = s->prho2;
     c.masks = s->masks;
     c.kept = s->kept;
+    c.kept_rg = s->kept_rg;
     c.kept_n = s->kept_n;
     c.boxes = s->boxes;
     c.mask_mode = MASK_TEST;
@@ -437,13 +438,17 @@ int ensure_masks(sph_system* s)
         s->mask_cap = cap;
     }
     // kept-candidate lists: 64 entries per chunk of a tile (T mask words)
+    // (with the candidates' ranges when a system is periodic)
     const long long kneed = (long long)(need / s->tile_T * 64);
-    if (kneed > s->kept_cap) {
+    if (kneed > s->kept_cap || (s->periodic && !s->kept_rg)) {
         dfree(s, s->kept, s->kept_cap);
+        dfree(s, s->kept_rg, s->kept_cap);
         s->kept_cap = 0;
         const long long cap = kneed + kneed / 4 + 4096;
-        if (dalloc(s, &s->kept, cap)) {
-            s->kept = nullptr;  // no lists: the loading passes stream and cull
+        if (dalloc(s, &s->kept, cap) || (s->periodic && dalloc(s, &s->kept_rg, cap))) {
+            dfree(s, s->kept, cap);  // no lists: the loading passes stream and cull
+            s->kept = nullptr;
+            s->kept_rg = nullptr;
             g_err.clear();
         } else {
             s->kept_cap = cap;
@@ -561,7 +566,7 @@ int destroy(sph_system* s)
     dfree(s, s->meta, META_SLOTS); dfree(s, s->pair_counter, 4); dfree(s, s->red_out, 4);
     dfree(s, s->queue, NQUEUE * QSTRIDE);
     dfree(s, s->masks, s->mask_cap); dfree(s, s->mask_total, 1);
-    dfree(s, s->kept, s->kept_cap);
+    dfree(s, s->kept, s->kept_cap); dfree(s, s->kept_rg, s->kept_cap);
     dfree(s, s->sysd, s->sysd_cap); dfree(s, s->sys_cnt, SYSCNT * s->sys_cnt_cap);
     if (s->meta_host) (void)hipHostFree(s->meta_host);
     if (s->sys_cnt_host) (void)hipHostFree(s->sys_cnt_host);

@@ -480,7 +480,8 @@ void enqueue_tiles(sph_system* s)
     k_tiles<<<nbc, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
                                         s->mask_start, ncells, s->tile_T,
                                         s->masks ? s->mask_cap : 0,
-                                        s->kept ? s->kept_cap : 0, s->tiles, s->meta,
+                                        s->kept && (s->kept_rg || !s->periodic) ? s->kept_cap : 0,
+                                        s->tiles, s->meta,
                                         s->mask_total);
     const double* c = s->cols[s->cur];
     const long long cap = s->cap;

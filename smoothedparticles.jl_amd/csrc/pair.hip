@@ -22,7 +22,9 @@
 // candidates (ballot + lane rank) to a ring of 128 LDS slots, and every 64
 // kept candidates form a chunk.  The kept order is the list order, and the
 // box and the cull test are the same in every pass, so all passes see the
-// same chunks.
+// same chunks.  The testing pass also records each tile's kept candidates
+// (and their ranges, for the image shifts of periodic systems); a pass that
+// loads its masks reads that list instead of streaming and culling again.
 //
 // Each chunk is handled in two phases:
 //   1. mask: lane (t, s) tests the chunk slots s + S*j (j < T) against
@@ -907,18 +909,17 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     if (!__any(act)) return;
     const double s_thr = d.g.s_thr;
     // a pass that loads the masks takes the kept candidates from the list the
-    // testing pass wrote (reference systems); otherwise it streams the
-    // candidate list and culls it again
-    const bool use_kept = !PERIODIC && load_masks && kept_ok;
-    const bool store_kept = !PERIODIC && store_masks && kept_ok;
+    // testing pass wrote; otherwise it streams the candidate list and culls
+    // it again
+    const bool use_kept = load_masks && kept_ok;
+    const bool store_kept = store_masks && kept_ok;
     const long long lbase = mbase / T * 64;
     const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
     int total = 0, nr = 0;
     Box<double> box{};
-    if (!use_kept) {
-        box = ld_const(reinterpret_cast<const Box<double>*>(c.boxes) + tile);
-        nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
-    }
+    if (!use_kept) box = ld_const(reinterpret_cast<const Box<double>*>(c.boxes) + tile);
+    // (periodic systems need the ranges' image shifts in either case)
+    if (!use_kept || PERIODIC) nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     __syncthreads();
     typename Op::Acc acc;
     Op::init(acc);
@@ -933,7 +934,9 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             const int k = head + lane;
             if (k < nk) {
                 const int q = c.kept[lbase + k];
-                Op::put(Op::template fetch<false>(c, q, Seg{}), k & 127, lds);
+                Seg sh{};
+                if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
+                Op::put(Op::template fetch<PERIODIC>(c, q, sh), k & 127, lds);
                 L.q[k & 127] = q;
             }
             fill = min(nk, head + 64);
@@ -951,7 +954,10 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             const int pos = compact(L, fill, nself, keep, rq, rrg, first);
             if (pos >= 0) {
                 Op::put(raw, pos & 127, lds);
-                if (store_kept) c.kept[lbase + pos] = rq;
+                if (store_kept) {
+                    c.kept[lbase + pos] = rq;
+                    if constexpr (PERIODIC) c.kept_rg[lbase + pos] = (unsigned char)rrg;
+                }
             }
             last = base + 64 >= total;
         }
@@ -1382,16 +1388,17 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     // the targets' records, read by the lanes that walk their pairs
     if (s == 0) tgts[t] = tg;
     // a pass that loads the masks takes the kept candidates from the list the
-    // testing pass wrote (reference systems); otherwise it streams the
-    // candidate list and culls it again, keeping everything within the guard
-    // band of the box
-    const bool use_kept = !PERIODIC && load_masks && kept_ok;
-    const bool store_kept = !PERIODIC && store_masks && kept_ok;
+    // testing pass wrote; otherwise it streams the candidate list and culls
+    // it again, keeping everything within the guard band of the box
+    const bool use_kept = load_masks && kept_ok;
+    const bool store_kept = store_masks && kept_ok;
     const long long lbase = mbase / T * 64;
     const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
     const float cull_thr = (float)(s_thr * (1.0 + BAND));
     int total = 0, nr = 0;
     Box<float> box{};
+    // (periodic systems need the ranges' image shifts in either case)
+    if (use_kept && PERIODIC) nr = build_cands<ROWS, PERIODIC>(d, c.cell_start, lc, L, total);
     if (!use_kept) {
         // RN(x - ox) is monotonic in x: the box of the relative positions
         // is the relative box
@@ -1415,7 +1422,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             const int k = head + lane;
             if (k < nk) {
                 const int q = c.kept[lbase + k];
-                Op::template put<FLAT>(Op::template fetch<false>(c, q, Seg{}), k & 127, ox, oy,
+                Seg sh{};
+                if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
+                Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & 127, ox, oy,
                                        oz, lds);
                 L.q[k & 127] = q;
             }
@@ -1435,7 +1444,10 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             const int pos = compact(L, fill, nself, keep, rq, rrg, first);
             if (pos >= 0) {
                 Op::template put<FLAT>(raw, pos & 127, ox, oy, oz, lds);
-                if (store_kept) c.kept[lbase + pos] = rq;
+                if (store_kept) {
+                    c.kept[lbase + pos] = rq;
+                    if constexpr (PERIODIC) c.kept_rg[lbase + pos] = (unsigned char)rrg;
+                }
             }
             last = base + 64 >= total;
         }

@@ -103,6 +103,7 @@ struct PairCtx {
     const int4* tiles;        // {global cell, first target, mask base lo, mask base hi}
     unsigned long long* masks;
     int* kept;                // kept candidates of each tile (64 per mask chunk)
+    unsigned char* kept_rg;   // their candidate ranges (periodic systems: image shift)
     int* kept_n;              // kept candidates per tile
     const double* boxes;      // bounding box of each tile's targets (6 doubles)
     int mask_mode;
@@ -309,6 +310,7 @@ struct sph_system {
     unsigned long long* masks;       // pair masks of the current cell list
     int* kept;                       // kept-candidate lists of the tiles (64 per chunk)
     long long kept_cap;              // entries allocated
+    unsigned char* kept_rg;          // kept_cap entries when a system is periodic, else null
     int* kept_n;                     // tile_cap: kept candidates per tile
     double* boxes;                   // 6 * tile_cap: bounding boxes of the tiles' targets
     long long mask_cap;              // words allocated
