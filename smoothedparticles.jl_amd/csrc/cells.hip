@@ -41,16 +41,35 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, long long n)
 // ---------------------------------------------------------------- prepare --
 // Per-system reference-index base of the inverse map and the count at the
 // start of the build.
-__global__ void k_sys_prep(int nsys, int* sc, int* meta)
+// One block of 256 threads; systems in rounds of 256, an exclusive scan of
+// the counts per round.
+__global__ __launch_bounds__(256) void k_sys_prep(int nsys, int* sc, int* meta)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int base = 0;
-    for (int s = 0; s < nsys; ++s) {
-        sc[SYSCNT * s + SC_NTOT] = sc[SYSCNT * s + SC_N];
-        sc[SYSCNT * s + SC_REFBASE] = base;
-        base += sc[SYSCNT * s + SC_N];
+    __shared__ int part[4];
+    int carry = 0;
+    for (int s0 = 0; s0 < nsys; s0 += 256) {
+        const int s = s0 + (int)threadIdx.x;
+        const int n = s < nsys ? sc[SYSCNT * s + SC_N] : 0;
+        int incl = n;
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) part[w] = incl;
+        __syncthreads();
+        int off = carry;
+        for (int k = 0; k < w; ++k) off += part[k];
+        const int round = part[0] + part[1] + part[2] + part[3];
+        if (s < nsys) {
+            sc[SYSCNT * s + SC_NTOT] = n;
+            sc[SYSCNT * s + SC_REFBASE] = off + incl - n;
+        }
+        carry += round;
+        __syncthreads();
     }
-    meta[META_NTOT] = meta[META_N];
+    if (threadIdx.x == 0) meta[META_NTOT] = meta[META_N];
 }
 
 // ------------------------------------------------------------------ keys --
@@ -501,7 +520,7 @@ void enqueue_build(sph_system* s)
     const unsigned nb = nblocks(s->n > 0 ? s->n : 1, 256);
     double* c = s->cols[s->cur];
     int* ic = s->icols[s->cur];
-    k_sys_prep<<<1, 1, 0, st>>>(nsys, s->sys_cnt, s->meta);
+    k_sys_prep<<<1, 256, 0, st>>>(nsys, s->sys_cnt, s->meta);
     k_fill_u32<<<nblocks(nbuckets, 256), 256, 0, st>>>(s->cnt, 0u, nbuckets);
     k_keys<<<nb, 256, 0, st>>>(s->sysd, nsys, ncells, c + C_X * cap, c + C_Y * cap,
                                c + C_Z * cap, ic + I_SYS * cap, s->meta, ic + I_KEY * cap,
