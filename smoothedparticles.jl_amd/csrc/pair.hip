@@ -1231,12 +1231,13 @@ struct Force32 {
         float A = t.pr + s.pos[jj].w;
         const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
         if (V & 1) {
+            // evaluated for every pair and selected, so the unrolled pair loop
+            // stays one basic block
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
-            if (vx < 0.0f) {
-                const float rhobar = 0.5f * (t.rho + v.w);
-                const float den = rhobar * (ss + (float)d.ph.eps);
-                A = fmaf((float)d.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
-            }
+            const float rhobar = 0.5f * (t.rho + v.w);
+            const float den = rhobar * (ss + (float)d.ph.eps);
+            const float Av = fmaf((float)d.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
+            A = vx < 0.0f ? Av : A;
         }
         const float f = u3 * A;
         q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);

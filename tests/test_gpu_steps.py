@@ -160,3 +160,21 @@ def test_full_size_3d_step0_mixed_vs_fp64(dr):
     assert n64 % 2 == 0  # the neighbour relation is symmetric
     r64, r32 = g64.get("rho"), g32.get("rho")
     assert np.max(np.abs(r32 / r64 - 1)) <= 5e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_density_pass_distance_tests(precision):
+    """The density pass's distance tests (sph_step_tests): every accepted
+    pair was tested, and the bounding-box cull tests fewer candidates than
+    the reference's 27-cell search visits."""
+    st = states.dam_break(3, 0.04)
+    p = st.params()
+    g, _ = load_dambreak(st, precision)
+    sp.step(g, 1, scheme="wcsph_se_rho0", **p)
+    dens_pairs, _ = g.step_pairs()
+    tests = g.step_tests()
+    sp.create_cell_list(g)  # positions moved by one step; the lists barely change
+    cand_all, _ = g.candidates()
+    assert 0 < dens_pairs <= tests
+    assert tests < cand_all
+    g.close()
