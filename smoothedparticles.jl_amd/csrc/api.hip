@@ -405,6 +405,7 @@ PairCtx make_ctx(sph_system* s)
     c.cell_start = s->cell_start;
     c.meta = s->meta;
     c.queue = s->queue;
+    c.qgrab = s->rows == 9 ? 1 : 4;
     c.cols = s->cols[s->cur];
     c.refidx = s->icols[s->cur] + I_REF * s->cap;
     c.prho2 = s->prho2;

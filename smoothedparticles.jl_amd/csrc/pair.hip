@@ -93,8 +93,12 @@ __device__ inline void two_sum_add(float& s, float& e, float x)
 
 // Tile queues.  The tile range [0, ntiles) is cut into NQUEUE contiguous
 // parts, one per queue head.  A wave starts on queue blockIdx % NQUEUE (the
-// XCD it runs on, so neighbouring tiles share that XCD's L2), takes QGRAB
+// XCD it runs on, so neighbouring tiles share that XCD's L2), takes qgrab
 // tiles per atomic, and when its queue is empty moves on to the next ones.
+// With one tile per grab the tiles in flight on an XCD are consecutive and
+// share their neighbour cells in the L2 (3D: 9.6 instead of 10.6 ms per
+// dam-break step); the short 2D tiles take four per grab, which spares
+// atomics.
 // Every wave leaves once all queues are past their ends.  The state is
 // wave-uniform (readfirstlane), so the tile record, the system id and the
 // system's parameters are scalar loads held in SGPRs.
@@ -117,11 +121,11 @@ __device__ inline int next_tile(const PairCtx& c, int ntiles, TileCursor& tc)
         const int lo = (int)((long long)ntiles * q / NQUEUE);
         const int hi = (int)((long long)ntiles * (q + 1) / NQUEUE);
         int start = 0;
-        if (__lane_id() == 0) start = atomicAdd(&c.queue[q * QSTRIDE], QGRAB);
+        if (__lane_id() == 0) start = atomicAdd(&c.queue[q * QSTRIDE], c.qgrab);
         start = __builtin_amdgcn_readfirstlane(start);
         if (start < hi - lo) {
             tc.cur = lo + start;
-            tc.end = min(hi, lo + start + QGRAB);
+            tc.end = min(hi, lo + start + c.qgrab);
             return tc.cur++;
         }
         ++tc.k;

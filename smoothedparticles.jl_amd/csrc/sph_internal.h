@@ -110,6 +110,7 @@ struct PairCtx {
     const int* cell_start;
     const int* meta;
     int* queue;               // NQUEUE tile-queue heads, QSTRIDE ints apart
+    int qgrab;                // tiles taken per queue atomic
     double* cols;             // current column buffer
     const int* refidx;
     const double* prho2;      // P/rho^2 per particle (force pass)
@@ -126,10 +127,9 @@ struct PairCtx {
 // Tile queues of the pair kernels: the tiles are split into NQUEUE
 // contiguous ranges with one head counter each (on its own 128-byte line);
 // workgroup b starts on queue b % NQUEUE (the XCD it runs on) and takes
-// QGRAB tiles per atomic, then helps the other queues.
+// PairCtx::qgrab tiles per atomic, then helps the other queues.
 static constexpr int NQUEUE = 8;
 static constexpr int QSTRIDE = 32;
-static constexpr int QGRAB = 4;
 
 // Binary operator ids inside the library.
 enum PairOp {
