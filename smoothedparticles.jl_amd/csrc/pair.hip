@@ -1110,7 +1110,7 @@ struct Density32 {
     static constexpr bool INTERACT = true;
     using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;  // pos.w = |x|^2
-    static constexpr int WAVES = 5;
+    static constexpr int WAVES = 6;  // a few spilled registers outside the loops
     struct Tgt {
         float x, y, z;
     };
