@@ -17,7 +17,7 @@
 //
 // Culling.  Only the candidates within reach of the tile -- squared
 // distance to the bounding box of its targets not above the predicate's
-// threshold -- can be accepted by any target, about 60% of the list in 3D.
+// threshold -- can be accepted by any target, about half of the list in 3D.
 // The list is streamed in 64-candidate steps; each step appends its kept
 // candidates (ballot + lane rank) to a ring of 128 LDS slots, and every 64
 // kept candidates form a chunk.  The kept order is the list order, and the
@@ -32,8 +32,9 @@
 //      after a density pass on the same cell list, loads that pass's mask.
 //      Interleaving the slices keeps the accepted counts of the lanes
 //      alike (neighbouring slots are neighbouring particles);
-//   2. interact: every lane walks the set bits of its mask, so the pair
-//      arithmetic runs once per accepted pair, not once per candidate.
+//   2. interact: the chunk's accepted pairs are cut into runs spread over
+//      all 64 lanes (balanced pairs, below), so the pair arithmetic runs
+//      once per accepted pair, not once per candidate.
 // At the end the S slices of a target are combined with wave shuffles.
 // The masks of target t for a chunk form one 64-bit word (the T-bit mask
 // of slice s in bits [s*T, s*T + T)), stored at
@@ -170,13 +171,6 @@ __device__ inline int lowest_bit(MaskT<T> m)
 {
     if constexpr (T > 32) return __ffsll((long long)m) - 1;
     else return __ffs(m) - 1;
-}
-
-template <int T>
-__device__ inline int bit_count(MaskT<T> m)
-{
-    if constexpr (T > 32) return __popcll(m);
-    else return __popc(m);
 }
 
 // ------------------------------------------------------------ candidates --
@@ -335,26 +329,6 @@ __device__ inline float box_dist2(const Box<float>& b, float x, float y, float z
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-// Load the stored mask word of target t for a chunk; slice s keeps its bits.
-template <int T>
-__device__ inline MaskT<T> load_mask(const PairCtx& c, long long mbase, int chunk, int t, int s)
-{
-    const unsigned long long word = c.masks[mbase + (long long)chunk * T + t];
-    if constexpr (T == 64) return word;
-    else return (MaskT<T>)((word >> (s * T)) & ((1ull << T) - 1ull));
-}
-
-// Assemble the 64-bit word of target t from its S slices; slice 0 stores it.
-template <int T>
-__device__ inline void store_mask(const PairCtx& c, long long mbase, int chunk, int t, int s,
-                                  MaskT<T> m)
-{
-    unsigned long long full = (unsigned long long)m << (T == 64 ? 0 : s * T);
-#pragma unroll
-    for (int o = T; o < 64; o <<= 1) full |= __shfl_xor(full, o);
-    if (s == 0) c.masks[mbase + (long long)chunk * T + t] = full;
-}
-
 // ------------------------------------------------------ balanced pairs --
 // The accepted pairs of a chunk are spread over all 64 lanes instead of
 // being walked by the lane that owns the target: with ~15% acceptance the
@@ -406,13 +380,6 @@ __device__ inline unsigned long long drop_low_bits(unsigned long long w, int n)
     c = (int)((unsigned)x & 1u);
     if (n >= c) pos += 1;
     return w & (~0ull << pos);
-}
-
-__device__ inline int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return __builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ inline int wave_max(int v)
