@@ -187,8 +187,13 @@ int sph_autotune(sph_system* s, const sph_params* params, int* best);
 /* Options (no counterpart in the reference).  SPH_OPTION_PAIR_MASKS (default
  * 1): a density pass records, per target and 64-candidate chunk, the set of
  * accepted candidates, and later pair passes on the same cell list read it
- * instead of re-testing candidates.  0 makes every pass test. */
+ * instead of re-testing candidates.  0 makes every pass test.
+ * SPH_OPTION_KEPT_LISTS (default 1): the density pass also records each
+ * tile's candidates that survived the bounding-box cull, and the passes that
+ * read the masks gather those instead of streaming and culling the whole
+ * candidate list again.  Either setting gives the same bits. */
 #define SPH_OPTION_PAIR_MASKS 0
+#define SPH_OPTION_KEPT_LISTS 1
 int sph_set_option(sph_system* s, int option, int value);
 
 /* Diagnostics (no counterpart in the reference): candidate particles the

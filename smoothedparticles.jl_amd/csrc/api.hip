@@ -410,7 +410,7 @@ PairCtx make_ctx(sph_system* s)
     c.refidx = s->icols[s->cur] + I_REF * s->cap;
     c.prho2 = s->prho2;
     c.masks = s->masks;
-    c.kept = s->kept;
+    c.kept = s->use_kept ? s->kept : nullptr;
     c.kept_rg = s->kept_rg;
     c.kept_n = s->kept_n;
     c.boxes = s->boxes;
@@ -524,6 +524,7 @@ int new_handle(int precision, sph_system** out)
     s->rows = 3;
     s->flat = 1;
     s->use_masks = true;
+    s->use_kept = true;
     int rc = 0;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -1233,6 +1234,10 @@ int sph_set_option(sph_system* s, int option, int value)
         s->use_masks = value != 0;
         s->masks_valid = false;
         if (s->use_masks && s->cells_valid) return ensure_masks(s);
+        return 0;
+    case SPH_OPTION_KEPT_LISTS:
+        s->use_kept = value != 0;
+        s->masks_valid = false;
         return 0;
     }
     return fail(SPH_EINVAL, "unknown option");

@@ -317,6 +317,7 @@ struct sph_system {
     unsigned long long* mask_total;  // device: words needed (1 value)
     bool masks_valid;                // masks match the cell list and positions
     bool use_masks;                  // option: reuse masks across passes
+    bool use_kept;                   // option: kept-candidate lists with the masks
     double* prho2;       // cap
     double* stage;       // 9 * cap (field IO)
     double* red_part;    // reduction partials

@@ -219,6 +219,12 @@ class ParticleSystem:
         passes on the same cell list (default on)."""
         check(lib().sph_set_option(self._h, N.SPH_OPTION_PAIR_MASKS, 1 if enable else 0))
 
+    def set_kept_lists(self, enable):
+        """Record the candidates that survive each tile's cull in the density
+        pass and gather only those in the passes that read its masks
+        (default on)."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_KEPT_LISTS, 1 if enable else 0))
+
     def step_pairs(self):
         """(density-pass pairs, force-pass pairs) of the last step() call."""
         d = ctypes.c_int64()

@@ -48,6 +48,7 @@ SPH_SCHEME_WCSPH_SE = 0
 SPH_SCHEME_WCSPH_SE_RHO0 = 1
 
 SPH_OPTION_PAIR_MASKS = 0
+SPH_OPTION_KEPT_LISTS = 1
 
 SPH_KERNEL_BUILD = 0
 SPH_KERNEL_DENSITY = 1

@@ -296,15 +296,17 @@ def test_autotune_keeps_state():
 @pytest.mark.parametrize("dim", [2, 3])
 def test_pair_masks_match_retesting(precision, dim):
     """Forces read from the density pass's accepted-candidate masks equal
-    the forces of a pass that re-tests every candidate, bit for bit."""
+    the forces of a pass that re-tests every candidate, bit for bit, with
+    and without the kept-candidate lists."""
     n = 3000 if dim == 2 else 6000
     x, lo, hi, rng = random_cloud(n, dim, 91 + dim)
     h = 0.06 if dim == 2 else 0.15
     v, rho, P, typ = fields(rng, n, dim)
     out = []
-    for masks in (True, False):
+    for masks, kept in ((True, True), (False, True), (True, False)):
         g, _ = make_both(x, h, lo, hi, precision, v, rho, P, typ)
         g.set_pair_masks(masks)
+        g.set_kept_lists(kept)
         sp.create_cell_list(g)
         sp.apply(g, sp.reset_rho)
         sp.apply(g, sp.find_rho, self=True, m=0.37)
@@ -312,5 +314,6 @@ def test_pair_masks_match_retesting(precision, dim):
         sp.apply(g, sp.reset_a)
         sp.apply(g, sp.internal_force, **dict(PHYS, hs=h / 2))
         out.append((g.get("rho"), g.get("a")))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        assert np.array_equal(out[0][1], o[1])
