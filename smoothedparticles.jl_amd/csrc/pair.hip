@@ -1138,7 +1138,7 @@ struct Force32 {
     static constexpr bool INTERACT = true;
     using Lds = Lds32Full;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
-    static constexpr int WAVES = 4;
+    static constexpr int WAVES = 5;  // spills a few registers; LDS fits five blocks too
     struct Tgt {
         float x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -1325,8 +1325,8 @@ __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
                               bool store_masks, bool kept_ok, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
-                              Bal<typename Op::Part>& bal, typename Op::Tgt* tgts,
-                              unsigned& npairs, unsigned long long& ntests)
+                              Bal<typename Op::Part>& bal, unsigned& npairs,
+                              unsigned long long& ntests)
 {
     const int4 td = ld_const(c.tiles + tile);
     const int cell = td.x, first = td.y;
@@ -1357,8 +1357,6 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const float thr_in = (float)(s_thr * (1.0 - BAND)) - p2;
     const float thr_out = (float)(s_thr * (1.0 + BAND)) - p2;
     const float mx = -2.0f * tg.x, my = -2.0f * tg.y, mz = -2.0f * tz;
-    // the targets' records, read by the lanes that walk their pairs
-    if (s == 0) tgts[t] = tg;
     // a pass that loads the masks takes the kept candidates from the list the
     // testing pass wrote; otherwise it streams the candidate list and culls
     // it again, keeping everything within the guard band of the box
@@ -1498,8 +1496,10 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
             if (tot > 0) {
                 typename Op::Part part;
                 Op::zero(part);
+                // the run's target record from its owner lane (every lane
+                // takes part in the shuffle)
+                const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
                 if (Op::INTERACT && cnt > 0) {
-                    const typename Op::Tgt ht = tgts[tt];
                     const unsigned long long w = drop_low_bits(bal.w[tt], r0);
                     const int hp = first + tt;
                     // the operator's uniform switches are taken outside the loop
@@ -1534,7 +1534,6 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair32(PairCtx c, int out_col
     __shared__ typename Op::Lds lds;
     __shared__ Cands<ROWS, PERIODIC> L;
     __shared__ Bal<typename Op::Part> bal;
-    __shared__ typename Op::Tgt tgts[T];
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
     const bool load_masks = ok && c.mask_mode == MASK_LOAD;
@@ -1547,7 +1546,7 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair32(PairCtx c, int out_col
     unsigned long long ntests = 0;
     for (int tile = next_tile(c, ntiles, tc); tile < ntiles; tile = next_tile(c, ntiles, tc))
         tile32<ROWS, FLAT, PERIODIC, T, Op>(c, tile, out_col, load_masks, store_masks, kept_ok, lds,
-                                            L, bal, tgts, npairs, ntests);
+                                            L, bal, npairs, ntests);
     if (c.pair_counter) count_pairs(c, npairs);
     count_tests(c, ntests);
 }
