@@ -415,6 +415,7 @@ PairCtx make_ctx(sph_system* s)
     c.kept_n = s->kept_n;
     c.boxes = s->boxes;
     c.mask_mode = MASK_TEST;
+    c.masks_mixed = s->precision == SPH_MIXED;
     return c;
 }
 

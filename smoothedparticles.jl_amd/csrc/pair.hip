@@ -1026,9 +1026,12 @@ __global__ __launch_bounds__(64, Op::WAVES) void k_pair64(PairCtx c)
     __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;
-    const bool load_masks = ok && c.mask_mode == MASK_LOAD;
-    const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     const bool kept_ok = ok && c.kept != nullptr && ld_const(c.meta + META_KEPT_OK) != 0;
+    // masks stored by a mixed pass number the candidates its FP32 cull kept;
+    // without that pass's kept lists this kernel's FP64 cull would keep
+    // others, so it tests instead
+    const bool load_masks = ok && c.mask_mode == MASK_LOAD && (kept_ok || !c.masks_mixed);
+    const bool store_masks = ok && c.mask_mode == MASK_TEST_STORE;
     TileCursor tc;
     tile_cursor_init(tc);
     // accepted pairs of all the wave's tiles, one atomic per wave

@@ -107,6 +107,7 @@ struct PairCtx {
     int* kept_n;              // kept candidates per tile
     const double* boxes;      // bounding box of each tile's targets (6 doubles)
     int mask_mode;
+    int masks_mixed;          // the masks were stored by a mixed-precision pass
     const int* cell_start;
     const int* meta;
     int* queue;               // NQUEUE tile-queue heads, QSTRIDE ints apart

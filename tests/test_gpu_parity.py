@@ -292,6 +292,33 @@ def test_autotune_keeps_state():
         assert np.array_equal(g.get(f), a)
 
 
+def test_fp64_operators_after_mixed_density_pass():
+    """In the mixed mode the operators without a mixed form run FP64 kernels;
+    after a mixed density pass they find the same pairs whether they read its
+    masks (with or without the kept-candidate lists) or re-test.  The FP64
+    and FP32 culls keep different candidates, so reading the mixed pass's
+    chunks sums in another order: equal to FP64 rounding, not bit for bit."""
+    n = 2500
+    x, lo, hi, rng = random_cloud(n, 2, 57)
+    v, rho, P, typ = fields(rng, n, 2)
+    kw = dict(PHYS, nu=1e-3)
+    out = []
+    for masks, kept in ((False, True), (True, True), (True, False)):
+        g, _ = make_both(x, 0.06, lo, hi, "mixed", v, rho, P, typ)
+        g.set_pair_masks(masks)
+        g.set_kept_lists(kept)
+        sp.create_cell_list(g)
+        sp.apply(g, sp.reset_rho)
+        sp.apply(g, sp.find_rho, self=True, m=0.37)
+        sp.apply(g, sp.reset_a)
+        sp.apply(g, sp.internal_force_rho0, **kw)
+        sp.apply(g, sp.balance_of_mass, **kw)
+        out.append((g.get("a"), g.get("Drho")))
+    for o in out[1:]:
+        for k in range(2):
+            assert np.max(np.abs(o[k] - out[0][k])) <= 1e-12 * np.abs(out[0][k]).max()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("dim", [2, 3])
 def test_pair_masks_match_retesting(precision, dim):
