@@ -577,6 +577,9 @@ struct Density64 {
     using Part = Acc;
     static __device__ void zero(Part& a) { a.w = 0.0; }
     static __device__ void fold(Acc& a, const Part& q) { a.w += q.w; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
                                     int, int, int, double, double, double, double ss, Acc& a)
     {
@@ -647,6 +650,13 @@ struct Force64 {
     {
         a.ax += q.ax; a.ay += q.ay; a.az += q.az; a.lx += q.lx; a.ly += q.ly; a.lz += q.lz;
     }
+    // variant bit 0: Monaghan viscosity (avk != 0); bit 1: laminar (mu != 0)
+    static constexpr int NVAR = 4;
+    static __device__ int variant(const SysDesc& d)
+    {
+        return (d.ph.avk != 0.0 ? 1 : 0) | (d.ph.mu != 0.0 ? 2 : 0);
+    }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -658,7 +668,7 @@ struct Force64 {
         const double2 v01 = s.v01[jj], v2a = s.v2a[jj];
         double A = t.pr + v2a.y;
         const double dvx = t.vx - v01.x, dvy = t.vy - v01.y, dvz = t.vz - v2a.x;
-        if (d.ph.avk != 0.0) {
+        if (V & 1) {
             double vx = dvx * dx + dvy * dy;
             vx = vx + dvz * dz;
             if (vx < 0.0) {
@@ -668,7 +678,7 @@ struct Force64 {
         }
         const double f = u3 * A;
         a.ax += f * dx; a.ay += f * dy; a.az += f * dz;
-        if (d.ph.mu != 0.0) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
+        if (V & 2) { a.lx += u3 * dvx; a.ly += u3 * dvy; a.lz += u3 * dvz; }
     }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&, Acc& a,
                                   int T, bool write)
@@ -723,6 +733,9 @@ struct ForceRho064 {
     using Part = Acc;
     static __device__ void zero(Part& a) { init(a); }
     static __device__ void fold(Acc& a, const Part& q) { a.ax += q.ax; a.ay += q.ay; a.az += q.az; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -788,6 +801,9 @@ struct Continuity64 {
     using Part = Acc;
     static __device__ void zero(Part& a) { a.d = 0.0; }
     static __device__ void fold(Acc& a, const Part& q) { a.d += q.d; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
                                     const Lds& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
@@ -842,6 +858,9 @@ struct Pairs64 {
     using Part = Acc;
     static __device__ void zero(Part&) {}
     static __device__ void fold(Acc&, const Part&) {}
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
     static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
                                     int, int p, int q, double, double, double, double, Acc&)
     {
@@ -854,6 +873,27 @@ struct Pairs64 {
     static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, Acc&, int,
                                   bool) {}
 };
+
+// A lane's run: cnt accepted slots of w (lowest first) against target ht.
+template <int V, bool FLAT, int T, class Op>
+__device__ inline void walk64(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
+                              const typename Op::Lds& lds, const int* q, unsigned long long w,
+                              int cnt, int rb, int hp, typename Op::Part& part)
+{
+    if constexpr (V < Op::NVAR) {
+        for (int k = 0; k < cnt; ++k) {
+            const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
+            w &= w - 1;
+            const int jj = rb + slot_of_bit<T>(b);
+            const double2 xy = lds.xy[jj];
+            const double dx = ht.x - xy.x, dy = ht.y - xy.y;
+            const double dz = FLAT ? 0.0 : ht.z - lds_z(lds, jj);
+            double ss = dx * dx + dy * dy;
+            if (!FLAT) ss = ss + dz * dz;
+            Op::template interact<V>(c, d, ht, lds, jj, hp, q[jj], dx, dy, dz, ss, part);
+        }
+    }
+}
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
@@ -988,18 +1028,14 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
                 Op::zero(part);
                 const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);
                 if (Op::INTERACT && cnt > 0) {
-                    unsigned long long w = drop_low_bits(bal.w[tt], r0);
-                    for (int k = 0; k < cnt; ++k) {
-                        const int b = __builtin_ctzll(w);  // w != 0: cnt <= its popcount
-                        w &= w - 1;
-                        const int jj = rb + slot_of_bit<T>(b);
-                        const double2 xy = lds.xy[jj];
-                        const double dx = ht.x - xy.x, dy = ht.y - xy.y;
-                        const double dz = FLAT ? 0.0 : ht.z - lds_z(lds, jj);
-                        double ss = dx * dx + dy * dy;
-                        if (!FLAT) ss = ss + dz * dz;
-                        Op::interact(c, d, ht, lds, jj, first + tt, L.q[jj], dx, dy, dz, ss,
-                                     part);
+                    const unsigned long long w = drop_low_bits(bal.w[tt], r0);
+                    const int hp = first + tt;
+                    // the operator's uniform switches are taken outside the loop
+                    switch (Op::variant(d)) {
+                    case 0: walk64<0, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    case 1: walk64<1, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    case 2: walk64<2, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
+                    default: walk64<3, FLAT, T, Op>(c, d, ht, lds, L.q, w, cnt, rb, hp, part); break;
                     }
                 }
                 bal.part[lane] = part;
