@@ -63,6 +63,30 @@ def test_pairs_random(precision, dim, n, h):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pairs_far_from_origin(precision, dim):
+    """A box far from the origin (coordinates ~1e3, ulp ~1e-13) holding a
+    lattice of spacing h/2, so many pairs sit at exactly h or within an ulp
+    of it: the tile cull and the mixed guard band must keep every pair the
+    FP64 predicate accepts."""
+    h = 0.1
+    rng = np.random.default_rng(5 + dim)
+    k = 14 if dim == 3 else 40
+    axes = [np.arange(k) * (h / 2)] * dim
+    g0 = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, dim)
+    jit = rng.integers(-2, 3, size=g0.shape) * np.spacing(1000.0)
+    x = np.zeros((len(g0), 3))
+    x[:, :dim] = 1000.0 + g0 + jit
+    lo = np.full(3, 999.9)
+    hi = np.array([1000.0 + k * h / 2 + 0.1] * 3)
+    if dim == 2:
+        lo[2] = hi[2] = 0.0
+    g, o = make_both(x, h, lo, hi, precision)
+    assert sp.create_cell_list(g) == o.create_cell_list() == len(x)
+    assert np.array_equal(sorted_pairs(sp.neighbour_pairs(g)), sorted_pairs(o.pairs()))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_pairs_ties_duplicates_boundaries(precision):
     h = 0.25
     pts = [[1.0, 1.0, 0], [1.25, 1.0, 0], [2.0, 2.0, 0], [np.nextafter(2.25, 3), 2.0, 0],
