@@ -113,3 +113,16 @@ def test_host_kernels_match_reference_constants():
     assert sp.wendland2(h, 0.0) == pytest.approx(7 / math.pi / h ** 2, rel=1e-15)
     assert sp.wendland3(h, 2 * h) == 0.0
     assert sp.rDwendland3(h, 0.0) == pytest.approx(-210 / math.pi / h ** 5, rel=1e-15)
+
+
+def test_header_constants_match_python():
+    """Every numeric #define of the C headers that the Python mirror also
+    names has the same value there."""
+    seen = 0
+    for h in ("sph.h", "sph_batch.h", "sph_setup.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        for name, val in re.findall(r"^#define\s+(SPH_[A-Z0-9_]+)\s+(-?\d+)\b", txt, re.M):
+            if hasattr(N, name):
+                assert getattr(N, name) == int(val), name
+                seen += 1
+    assert seen >= 10
