@@ -512,6 +512,16 @@ class Batch:
     def set_tile(self, T):
         check(lib().sph_batch_set_tile(self._h, int(T)))
 
+    def set_pair_masks(self, enable):
+        """Reuse the density pass's accepted-candidate masks (default on); a
+        batch handle is a handle of sph.h, so sph_set_option applies."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_PAIR_MASKS, 1 if enable else 0))
+
+    def set_kept_lists(self, enable):
+        """Gather only the candidates each tile's cull kept in the passes that
+        read the masks (default on)."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_KEPT_LISTS, 1 if enable else 0))
+
     def autotune(self):
         best = ctypes.c_int()
         check(lib().sph_batch_autotune(self._h, ctypes.byref(best)))

@@ -98,6 +98,27 @@ def test_retire_and_join_between_steps():
     b.close()
 
 
+def test_batch_options_keep_bits():
+    """A batch with periodic and reference systems gives the same bits with
+    the mask reuse and kept-candidate lists on or off."""
+    print("batch options", flush=True)
+    sts = small_states()
+    out = []
+    for masks, kept in ((True, True), (True, False), (False, True)):
+        b = sp.Batch("mixed")
+        b.set_tile(TILE)
+        b.set_pair_masks(masks)
+        b.set_kept_lists(kept)
+        ids = [b.add_state(st, periodic=per) for st, per in sts]
+        b.init_rho0()
+        b.step(8)
+        out.append([b.get(i, f) for i in ids for f in ("x", "v", "rho")])
+        b.close()
+    for o in out[1:]:
+        for a, c in zip(out[0], o):
+            assert np.array_equal(a, c)
+
+
 def test_retire_two_systems_at_once():
     """Two systems retired before one build share the dead bucket (their
     reference indices overlap); the survivor is unaffected."""
