@@ -12,6 +12,7 @@
  *   apply!(sys, op!; self)            src/core.jl:151-161    -> sph_apply
  *   sum(p -> energy(p), particles)    tests/test_collision_2d.jl:137,
  *                                     examples/collapse_dry.jl:173-187 -> sph_reduce
+ *   SmoothedParticles.sum(sys, f, x)  src/core.jl:240-258    -> sph_probe_sum
  *
  * Conventions
  *   - Every function returns 0 on success and a negative SPH_E* code on
@@ -150,6 +151,22 @@ int sph_apply(sph_system* s, int op, const sph_params* params, int self);
 
 /* Global reductions; out receives 1 or 3 doubles. */
 int sph_reduce(sph_system* s, int what, const sph_params* params, double* out);
+
+/* SmoothedParticles.sum(sys, func, x) (src/core.jl:240-258) at np probe
+ * points x[np][3] (host), with the interpolation closures of
+ * examples/cavity_flow.jl:166-172.  For probe i, out[4i] receives the sum of
+ * (phi*m)*W(h,r) and out[4i+1+k] the sum of ((phi*m)*v_k)*W(h,r) over the
+ * particles the reference visits from x_i (its find_key cell and the
+ * key_diff neighbours, in that order) whose distance r does not exceed h;
+ * W is wendland2 in 2D and wendland3 in 3D, m = params->m, and phi is
+ * Float64(type == FLUID) with SPH_PROBE_FLUID_ONLY, else 1.  The velocity
+ * interpolant is out[4i+1+k] / out[4i].  Sums are FP64 in the reference's
+ * order in either precision mode.  As in the reference, a probe with a NaN
+ * coordinate gets key -1 and a NaN distance is not rejected.  Needs a
+ * current cell list; periodic systems: SPH_EUNSUP. */
+#define SPH_PROBE_FLUID_ONLY 1
+int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const double* x,
+                  int64_t np, double* out);
 
 /* Neighbour pairs of the current cell list as reference particle indices
  * (p, q), p != q, dist <= h: pairs[2k], pairs[2k+1].  The order is

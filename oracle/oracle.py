@@ -50,6 +50,8 @@ def lib():
             "orc_create_cell_list": (ctypes.c_int64, [ctypes.c_void_p]),
             "orc_neighbour_pairs": (ctypes.c_int64, [ctypes.c_void_p, _pi, ctypes.c_int64]),
             "orc_neighbour_counts": (ctypes.c_int64, [ctypes.c_void_p, _pi]),
+            "orc_probe_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _pd,
+                                             ctypes.c_int64, _pd]),
             "orc_brute_pairs": (ctypes.c_int64, [_pd, ctypes.c_int64, ctypes.c_double, _pi,
                                                  ctypes.c_int64]),
             "orc_density_sum": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int]),
@@ -174,6 +176,17 @@ class OracleSystem:
         c = np.empty(max(self.n, 1), dtype=np.int64)
         tot = lib().orc_neighbour_counts(self._s, c.ctypes.data_as(_pi))
         return c[: self.n], tot
+
+    def probe_sum(self, x, m, fluid_only=False):
+        """SmoothedParticles.sum at probe points x (np, 3): (np, 4) array of
+        sum (phi m) W and sum (phi m) v_k W (src/core.jl:240-258)."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 3))
+        out = np.zeros((x.shape[0], 4), dtype=np.float64)
+        rc = lib().orc_probe_sum(self._s, float(m), 1 if fluid_only else 0,
+                                 x.ctypes.data_as(_pd), x.shape[0], out.ctypes.data_as(_pd))
+        if rc != 0:
+            raise RuntimeError("orc_probe_sum: no current cell list or periodic system")
+        return out
 
     def density_sum(self, self_term=True, **kw):
         p = params(**kw)

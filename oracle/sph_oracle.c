@@ -505,6 +505,41 @@ int64_t orc_neighbour_counts(const orc_sys* s, int64_t* counts)
     return tot;
 }
 
+/* SmoothedParticles.sum(sys, func, x), src/core.jl:240-258, at np probe
+ * points x[np][3], with the interpolation closures of
+ * examples/cavity_flow.jl:166-172: out[4i] = sum of (phi*m)*W(h,r) and
+ * out[4i+1+k] = sum of ((phi*m)*v_k)*W(h,r), phi = Float64(type == FLUID)
+ * when fluid_only, else 1.  Cells in key_diff order behind the linear bounds
+ * test (:243-245), each cell in its stored order (:246-250), r = norm(x - q.x)
+ * (:251), rejected only when r > h (:252), so a NaN r is summed.  There is
+ * no p == q exclusion.  Reference (non-periodic) systems only; returns -1
+ * without a current cell list or for a periodic system. */
+int orc_probe_sum(const orc_sys* s, double m, int fluid_only, const double* x, int64_t np,
+                  double* out)
+{
+    if (!s->cells_valid || s->periodic) return -1;
+    for (int64_t i = 0; i < np; ++i) {
+        const double* xp = x + 3 * i;
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        int64_t key = find_key(s, xp);
+        for (int t = 0; t < s->n_diff; ++t) {
+            int64_t nk = key + s->key_diff[t];
+            if (!(1 <= nk && nk <= s->key_max)) continue;
+            for (int64_t e = s->cell_start[nk - 1]; e < s->cell_start[nk]; ++e) {
+                int64_t q = s->cell_idx[e];
+                double r = dist3(xp, s->x + 3 * q);
+                if (r > s->h) continue;
+                double w = (fluid_only ? (s->type[q] == 0.0 ? 1.0 : 0.0) : 1.0) * m;
+                double k = W(s, r);
+                a[0] = a[0] + w * k;
+                for (int d = 0; d < 3; ++d) a[1 + d] = a[1 + d] + (w * s->v[3 * q + d]) * k;
+            }
+        }
+        for (int d = 0; d < 4; ++d) out[4 * i + d] = a[d];
+    }
+    return 0;
+}
+
 /* O(N^2) brute force with the same FP64 predicate and no cells: all ordered
  * pairs p != q with dist <= h, p ascending then q ascending. */
 int64_t orc_brute_pairs(const double* x, int64_t n, double h, int64_t* out, int64_t cap)

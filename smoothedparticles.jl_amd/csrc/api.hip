@@ -23,6 +23,8 @@ void enqueue_unary(sph_system* s, int op, double* prho2);
 void enqueue_prho2(sph_system* s);
 void enqueue_kick_drift(sph_system* s);
 void enqueue_reduce(sph_system* s, int sys, long long off, long long n, int what, double* out_dev);
+void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long long np, double m,
+                       int fluid_only, double* out_dev);
 void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,
                        double* out_dev);
 void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,
@@ -867,6 +869,36 @@ int reduce_slot(sph_system* s, int slot, int what, double* out)
     return 0;
 }
 
+// Probe sums of system `slot` (SmoothedParticles.sum at points,
+// src/core.jl:240-258).  The probe and result staging is allocated per call:
+// the call runs at frame cadence, not in the step loop.
+int probe_sum(sph_system* s, int slot, double m, int flags, const double* x, long long np,
+              double* out)
+{
+    if (np < 0 || np > (1LL << 28)) return fail(SPH_EINVAL, "probe count out of range");
+    if (flags & ~SPH_PROBE_FLUID_ONLY) return fail(SPH_EINVAL, "unknown probe flags");
+    if (s->sys_host[slot].g.periodic)
+        return fail(SPH_EUNSUP, "probe sums of periodic systems are not supported");
+    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    if (np == 0) return 0;
+    double* buf = nullptr;
+    HIPC(hipMalloc((void**)&buf, sizeof(double) * 7 * (size_t)np));
+    double* pin = buf;
+    double* pout = buf + 3 * np;
+    hipError_t e = hipMemcpyAsync(pin, x, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) {
+        enqueue_probe_sum(s, slot, pin, np, m, (flags & SPH_PROBE_FLUID_ONLY) ? 1 : 0, pout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, pout, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    const hipError_t ef = hipFree(buf);
+    if (e != hipSuccess) return fail(SPH_EHIP, std::string("probe sum: ") + hipGetErrorString(e));
+    if (ef != hipSuccess) return fail(SPH_EHIP, std::string("hipFree: ") + hipGetErrorString(ef));
+    return 0;
+}
+
 // Neighbour pairs of the current cell lists; with only_slot >= 0, of that
 // system only (the pair kernels skip the tiles of the others).
 int neighbour_pairs(sph_system* s, int only_slot, int64_t* pairs, int64_t cap, int64_t* n)
@@ -1131,6 +1163,14 @@ int sph_reduce(sph_system* s, int what, const sph_params* params, double* out)
     int rc = set_params0(s, params);
     if (rc) return rc;
     return reduce_slot(s, 0, what, out);
+}
+
+int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const double* x,
+                  int64_t np, double* out)
+{
+    if (!s || !params || (np > 0 && (!x || !out))) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return probe_sum(s, 0, params->m, flags, x, np, out);
 }
 
 int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)

@@ -202,6 +202,95 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nb
     }
 }
 
+// ----------------------------------------------------------- probe sums --
+// SmoothedParticles.sum(sys, func, x) (src/core.jl:240-258) for a batch of
+// probe points, with the interpolation closures of
+// examples/cavity_flow.jl:166-172.  One lane per probe visits the cells in
+// key_diff order (di outermost, src/structs.jl:70-82) and each cell in its
+// stored (descending reference index) order, so every sum is accumulated in
+// the reference's order, in FP64 without contraction: the results are
+// bit-exact.  This runs at frame cadence (a few hundred probes), so the
+// lane-per-probe walk, latency-bound on the cell ranges, is not worth
+// splitting across a wave at the cost of that order.
+
+// find_key (src/structs.jl:97-106): -1 when Int64(floor(x/h)) throws, i.e.
+// for a NaN or infinite coordinate or one outside the Int64 range.
+__device__ inline bool probe_floor(double v, double h, long long* out)
+{
+    const double f = floor(v / h);
+    if (!(f >= -9223372036854775808.0 && f < 9223372036854775808.0)) return false;
+    *out = (long long)f;
+    return true;
+}
+
+__device__ inline long long probe_key(const Geo& g, double x, double y, double z)
+{
+    long long fi, fj, fk;
+    if (!probe_floor(x, g.h, &fi) || !probe_floor(y, g.h, &fj) || !probe_floor(z, g.h, &fk))
+        return -1;
+    const long long i = 1 + fi - g.kp[0], j = 1 + fj - g.kp[1], k = 1 + fk - g.kp[2];
+    return i + (long long)g.L1 * (j - 1) + (long long)g.L1 * g.L2 * (k - 1);
+}
+
+// wendland2 / wendland3 (src/kernels.jl:108-115, :156-163) by dimension.
+__device__ inline double probe_w(const Geo& g, double r)
+{
+    const double x = r / g.h;
+    if (x > 1.0) return 0.0;
+    const double u = 1.0 - x, u2 = u * u, u4 = u2 * u2;
+    if (g.dim == 2) return 2.228169203286535 * u4 * (1.0 + 4.0 * x) / (g.h * g.h);
+    return 3.3422538049298023 * u4 * (1.0 + 4.0 * x) / (g.h * g.h * g.h);
+}
+
+// out[4i] = sum of (phi m) W(h, r), out[4i+1+k] = sum of ((phi m) v_k) W(h, r)
+// over the particles visited from probe i whose r > h is false (a NaN r is
+// kept, as in the reference); phi = Float64(type == FLUID) with fluid_only,
+// else 1.
+__global__ __launch_bounds__(256) void k_probe_sum(Geo g, int cell_base,
+                                                   const int* __restrict__ cell_start,
+                                                   const double* __restrict__ cols, long long cap,
+                                                   const double* __restrict__ probes, int np,
+                                                   double m, int fluid_only,
+                                                   double* __restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= np) return;
+    const double x = probes[3 * i], y = probes[3 * i + 1], z = probes[3 * i + 2];
+    const long long key = probe_key(g, x, y, z);
+    const long long L1 = g.L1, L12 = (long long)g.L1 * g.L2;
+    const int kr = g.L3 == 1 ? 0 : 1;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -kr; dk <= kr; ++dk) {
+                const long long nk = key + (di + L1 * dj + L12 * dk);
+                if (nk < 1 || nk > g.key_max) continue;  // src/core.jl:244
+                const int c = cell_base + (int)(nk - 1);
+                const int e1 = cell_start[c + 1];
+                for (int e = cell_start[c]; e < e1; ++e) {
+                    const double d1 = x - UCOL(cols, cap, C_X, e);
+                    const double d2 = y - UCOL(cols, cap, C_Y, e);
+                    const double d3 = z - UCOL(cols, cap, C_Z, e);
+                    double s = d1 * d1 + d2 * d2;
+                    s = s + d3 * d3;
+                    const double r = sqrt(s);
+                    if (r > g.h) continue;
+                    const double phi =
+                        fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
+                    const double w = phi * m;
+                    const double k = probe_w(g, r);
+                    a0 = a0 + w * k;
+                    a1 = a1 + (w * UCOL(cols, cap, C_VX, e)) * k;
+                    a2 = a2 + (w * UCOL(cols, cap, C_VY, e)) * k;
+                    a3 = a3 + (w * UCOL(cols, cap, C_VZ, e)) * k;
+                }
+            }
+    out[4 * i] = a0;
+    out[4 * i + 1] = a1;
+    out[4 * i + 2] = a2;
+    out[4 * i + 3] = a3;
+}
+
 // ------------------------------------------------------------ field I/O --
 // Host order is the system's reference order: host slot refidx[k] <->
 // device slot k, for k in the system's range [off, off + n).
@@ -305,6 +394,16 @@ void enqueue_reduce(sph_system* s, int sys, long long off, long long n, int what
         k_reduce_part<<<nb, 256, 0, s->stream>>>(what, (int)off, (int)n, s->cols[s->cur], s->cap,
                                                  s->sysd, sys, s->red_part);
     k_reduce_final<<<1, 256, 0, s->stream>>>(s->red_part, nb, out_dev);
+}
+
+void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long long np, double m,
+                       int fluid_only, double* out_dev)
+{
+    if (np <= 0) return;
+    const SysDesc& d = s->sys_host[sys];
+    k_probe_sum<<<nblk(np), 256, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
+                                                 s->cols[s->cur], s->cap, probes_dev, (int)np, m,
+                                                 fluid_only, out_dev);
 }
 
 void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,

@@ -34,6 +34,7 @@ __all__ = [
     "internal_force_rho0", "balance_of_mass", "reset_rho", "reset_a", "find_pressure",
     "find_pressure_rho0", "find_pressure_continuity", "accelerate", "move",
     "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
+    "probe_sums",
 ]
 
 FLUID = 0.0
@@ -354,6 +355,22 @@ def neighbour_pairs(sys):
     check(lib().sph_neighbour_pairs(
         sys._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), total, ctypes.byref(n)))
     return out[:total]
+
+
+def probe_sums(sys, x, fluid_only=False, **params):
+    """SmoothedParticles.sum(sys, func, x) (src/core.jl:240-258) at probe
+    points x (shape (np, 3) or (3,)) for the interpolation closures of
+    examples/cavity_flow.jl:166-172.  Returns shape (np, 4): column 0 is the
+    sum of (phi*m)*W(h,r) (the Shepard denominator gamma), columns 1..3 the
+    sums of ((phi*m)*v_k)*W(h,r), with phi = Float64(type == FLUID) when
+    fluid_only, else 1, and m = params["m"].  Needs a current cell list."""
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 3))
+    out = np.zeros((xs.shape[0], 4), dtype=np.float64)
+    p = make_params(**params)
+    flags = N.SPH_PROBE_FLUID_ONLY if fluid_only else 0
+    check(lib().sph_probe_sum(sys._h, ctypes.byref(p), flags, xs.ctypes.data_as(_dptr),
+                              xs.shape[0], out.ctypes.data_as(_dptr)))
+    return out
 
 
 def step(sys, nsteps, scheme="wcsph_se", **params):

@@ -49,6 +49,7 @@ SPH_SCHEME_WCSPH_SE_RHO0 = 1
 
 SPH_OPTION_PAIR_MASKS = 0
 SPH_OPTION_KEPT_LISTS = 1
+SPH_PROBE_FLUID_ONLY = 1
 
 SPH_KERNEL_BUILD = 0
 SPH_KERNEL_DENSITY = 1
@@ -103,6 +104,7 @@ SIGNATURES = {
     "sph_apply": (_i, [_p, _i, ctypes.POINTER(sph_params), _i]),
     "sph_reduce": (_i, [_p, _i, ctypes.POINTER(sph_params), _pd]),
     "sph_neighbour_pairs": (_i, [_p, _pi64, _i64, _pi64]),
+    "sph_probe_sum": (_i, [_p, ctypes.POINTER(sph_params), _i, _pd, _i64, _pd]),
     "sph_step": (_i, [_p, _i, ctypes.POINTER(sph_params), _i, _pi64]),
     "sph_step_pairs": (_i, [_p, _pi64, _pi64]),
     "sph_step_tests": (_i, [_p, _pi64]),

@@ -1,0 +1,106 @@
+"""Probe sums on the GPU (sph_probe_sum, SmoothedParticles.sum at points,
+src/core.jl:240-258) against the oracle.  Both walk the cells in key_diff
+order and each cell in its stored order and evaluate the same FP64
+expressions without contraction, so the comparison is bit-exact (NaN where
+the reference gives NaN), in either precision mode of the handle."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import smoothedparticles_hip as sp
+
+pytestmark = pytest.mark.gpu
+
+PHYS = dict(m=0.4, rho0=1000.0, c=50.0, alpha=0.02, hs=0.05, mu=0.0, nu=0.0,
+            g=(0.0, -9.8, 0.0), dt=1e-4)
+
+
+def make_both(x, h, lo, hi, precision, v, typ):
+    g = sp.ParticleSystem(sp.Box(*lo, *hi), h, precision)
+    g.add_particles(x, v=v, type=typ)
+    o = O.OracleSystem(h, np.asarray(lo, float), np.asarray(hi, float))
+    o.load(x, v, None, None, typ)
+    return g, o
+
+
+def probes(rng, x, h, dim, hi):
+    P = rng.random((300, 3)) * hi * 1.1 - 0.05 * hi   # inside and just outside the box
+    P[:40] = x[:40]                                  # at particles
+    P[40:60] = np.round(P[40:60] / h) * h            # on cell faces and corners
+    P[60] = [math.nan, 0.5, 0.0]
+    P[61] = [math.inf, 0.5, 0.0]
+    P[62] = [0.5, -math.inf, 0.0]
+    P[63] = [0.2, math.nan, math.nan]
+    P[64] = [5.0, 5.0, 5.0]                          # far outside
+    if dim == 2:
+        P[:60, 2] = 0.0
+        P[65:250, 2] = 0.0                           # the rest probe off the plane
+    return P
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("dim,n,h", [(2, 3000, 0.05), (3, 4000, 0.12), (2, 1500, 0.3)])
+def test_probe_sums_bit_exact(precision, dim, n, h):
+    rng = np.random.default_rng(31 + n)
+    x = rng.random((n, 3))
+    if dim == 2:
+        x[:, 2] = 0.0
+    hi = np.array([1.0, 1.0, 1.0 if dim == 3 else 0.0])
+    v = rng.normal(size=(n, 3))
+    typ = (rng.random(n) < 0.25).astype(float)
+    g, o = make_both(x, h, np.zeros(3), hi, precision, v, typ)
+    assert sp.create_cell_list(g) == o.create_cell_list() == n
+    P = probes(rng, x, h, dim, np.where(hi > 0, hi, 1.0))
+    for fluid_only in (False, True):
+        got = sp.probe_sums(g, P, fluid_only=fluid_only, **PHYS)
+        ref = o.probe_sum(P, PHYS["m"], fluid_only)
+        assert np.array_equal(got, ref, equal_nan=True), (fluid_only, np.nanmax(abs(got - ref)))
+    assert np.isnan(ref[60, 0]) or np.isnan(ref[63, 0]) or h > 0.2
+    assert np.all(got[61:63] == 0.0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_probe_sums_after_steps(precision):
+    """After device-resident steps and a rebuild, the probes read the moved
+    particles through the new cell list; the oracle is loaded with the
+    GPU's state, so the comparison stays bit-exact."""
+    rng = np.random.default_rng(4)
+    n, h = 2500, 0.05
+    x = rng.random((n, 3)) * 0.8 + 0.1
+    x[:, 2] = 0.0
+    hi = np.array([1.0, 1.0, 0.0])
+    g = sp.ParticleSystem(sp.Box(0, 0, 0, 1, 1, 0), h, precision)
+    g.add_particles(x, v=np.zeros((n, 3)), type=(rng.random(n) < 0.2).astype(float))
+    sp.step(g, 5, **PHYS)
+    sp.create_cell_list(g)
+    xs = g.get("x")
+    o = O.OracleSystem(h, np.zeros(3), hi)
+    o.load(xs, g.get("v"), None, None, g.get("type"))
+    assert o.create_cell_list() == len(xs)
+    P = probes(rng, xs, h, 2, np.array([1.0, 1.0, 1.0]))
+    got = sp.probe_sums(g, P, fluid_only=True, **PHYS)
+    ref = o.probe_sum(P, PHYS["m"], True)
+    assert np.array_equal(got, ref, equal_nan=True)
+
+
+def test_probe_sums_state_and_errors():
+    """No current cell list: SPH_ESTATE; unknown flags: SPH_EINVAL; zero
+    probes: nothing to do."""
+    import ctypes
+    from smoothedparticles_hip import _native as N
+    rng = np.random.default_rng(2)
+    x = rng.random((500, 3))
+    x[:, 2] = 0.0
+    g = sp.ParticleSystem(sp.Box(0, 0, 0, 1, 1, 0), 0.1, "fp64")
+    g.add_particles(x)
+    with pytest.raises(sp.SPHError):
+        sp.probe_sums(g, [[0.5, 0.5, 0.0]], **PHYS)
+    sp.create_cell_list(g)
+    assert sp.probe_sums(g, np.zeros((0, 3)), **PHYS).shape == (0, 4)
+    p = sp.make_params(**PHYS)
+    buf = (ctypes.c_double * 4)()
+    pt = (ctypes.c_double * 3)(0.5, 0.5, 0.0)
+    assert N.lib().sph_probe_sum(g._h, ctypes.byref(p), 2, pt, 1, buf) == N.SPH_EINVAL
+    assert N.lib().sph_probe_sum(g._h, ctypes.byref(p), 0, pt, -1, buf) == N.SPH_EINVAL
