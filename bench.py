@@ -103,6 +103,7 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
     g.profile(False)
     sp.create_cell_list(g)
     cand_all, cand_fluid = g.candidates()
+    probe = probe_leg(sp, g, st, p)
     n = len(g)
     g.close()
     el = t1 - t0
@@ -111,8 +112,32 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
         force_ms=force_ms / max(force_n, 1), density_ms=dens_ms / max(dens_n, 1),
         build_ms=build_ms / max(build_n, 1), force_pairs_per_pass=force_pairs / steps,
         density_pairs_per_pass=dens_pairs / steps, tests_per_pass=tests / steps,
-        cand_fluid=cand_fluid, cand_all=cand_all,
+        cand_fluid=cand_fluid, cand_all=cand_all, probe=probe,
     )
+
+
+PROBES = 1 << 16
+
+
+def probe_leg(sp, g, st, p, reps=5):
+    """SmoothedParticles.sum at PROBES seeded points in the fluid's bounding
+    box (Shepard denominator and velocity sums, fluid only), timed per call
+    with host buffers: staging, both PCIe copies and the kernel."""
+    import numpy as np
+
+    rng = np.random.default_rng(1)
+    xf = st.x[st.type == 0.0]
+    lo, hi = xf.min(axis=0), xf.max(axis=0)
+    P = lo + rng.random((PROBES, 3)) * (hi - lo)
+    sp.probe_sums(g, P, fluid_only=True, **p)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = sp.probe_sums(g, P, fluid_only=True, **p)
+    el = (time.perf_counter() - t0) / reps
+    filled = out[:, 0] > 0
+    return {"probes": PROBES, "call_ms": 1e3 * el, "probes_per_s": PROBES / el,
+            "probes_with_fluid": int(filled.sum()),
+            "timing": "per call with host buffers (staging, PCIe both ways, kernel)"}
 
 
 def accuracy(st, checkpoints=(100, 1000)):
@@ -385,6 +410,7 @@ def main():
         "roofline": roofline(dominant),
         "roofline_" + secondary: roofline(secondary),
     }
+    out["probe_sums"] = res["probe"]
     if other is not None:
         out["fp64_value"] = other["value"]
         out["mixed_over_fp64"] = res["value"] / other["value"]

@@ -205,13 +205,14 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nb
 // ----------------------------------------------------------- probe sums --
 // SmoothedParticles.sum(sys, func, x) (src/core.jl:240-258) for a batch of
 // probe points, with the interpolation closures of
-// examples/cavity_flow.jl:166-172.  One lane per probe visits the cells in
-// key_diff order (di outermost, src/structs.jl:70-82) and each cell in its
-// stored (descending reference index) order, so every sum is accumulated in
-// the reference's order, in FP64 without contraction: the results are
-// bit-exact.  This runs at frame cadence (a few hundred probes), so the
-// lane-per-probe walk, latency-bound on the cell ranges, is not worth
-// splitting across a wave at the cost of that order.
+// examples/cavity_flow.jl:166-172.  One wave per probe: the wave visits the
+// cells in key_diff order (di outermost, src/structs.jl:70-82) and each cell
+// in its stored (descending reference index) order, as one concatenated list
+// taken 64 candidates at a time.
+// The lanes test their candidate and form its four terms in parallel; the
+// accepted terms are then added in lane order, each read with readlane, so
+// every sum is accumulated in the reference's order, in FP64 without
+// contraction: the results are bit-exact.
 
 // find_key (src/structs.jl:97-106): -1 when Int64(floor(x/h)) throws, i.e.
 // for a NaN or infinite coordinate or one outside the Int64 range.
@@ -242,10 +243,19 @@ __device__ inline double probe_w(const Geo& g, double r)
     return 3.3422538049298023 * u4 * (1.0 + 4.0 * x) / (g.h * g.h * g.h);
 }
 
+// The double in lane j of a wave (j wave-uniform).
+__device__ inline double readlane_f64(double v, int j)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // out[4i] = sum of (phi m) W(h, r), out[4i+1+k] = sum of ((phi m) v_k) W(h, r)
 // over the particles visited from probe i whose r > h is false (a NaN r is
 // kept, as in the reference); phi = Float64(type == FLUID) with fluid_only,
-// else 1.
+// else 1.  Block of 4 waves, probe i = 4 * block + wave.
 __global__ __launch_bounds__(256) void k_probe_sum(Geo g, int cell_base,
                                                    const int* __restrict__ cell_start,
                                                    const double* __restrict__ cols, long long cap,
@@ -253,42 +263,82 @@ __global__ __launch_bounds__(256) void k_probe_sum(Geo g, int cell_base,
                                                    double m, int fluid_only,
                                                    double* __restrict__ out)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= np) return;
+    const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    if (i >= np) return;  // wave-uniform
     const double x = probes[3 * i], y = probes[3 * i + 1], z = probes[3 * i + 2];
     const long long key = probe_key(g, x, y, z);
     const long long L1 = g.L1, L12 = (long long)g.L1 * g.L2;
     const int kr = g.L3 == 1 ? 0 : 1;
+    // Lane t < ncell holds the range of the t-th key_diff cell (di outermost,
+    // dk innermost, src/structs.jl:70-82) if it passes the bounds test of
+    // src/core.jl:244, else an empty range; a wave prefix sum over the range
+    // lengths concatenates them in that order.
+    const int ncell = kr ? 27 : 9, nd = kr ? 3 : 1;
+    int my_e0 = 0, my_cnt = 0;
+    if (lane < ncell) {
+        const int di = lane / (3 * nd) - 1, dj = (lane / nd) % 3 - 1, dk = kr ? lane % 3 - 1 : 0;
+        const long long nk = key + (di + L1 * dj + L12 * dk);
+        if (nk >= 1 && nk <= g.key_max) {
+            const int c = cell_base + (int)(nk - 1);
+            my_e0 = cell_start[c];
+            my_cnt = cell_start[c + 1] - my_e0;
+        }
+    }
+    int incl = my_cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int di = -1; di <= 1; ++di)
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int dk = -kr; dk <= kr; ++dk) {
-                const long long nk = key + (di + L1 * dj + L12 * dk);
-                if (nk < 1 || nk > g.key_max) continue;  // src/core.jl:244
-                const int c = cell_base + (int)(nk - 1);
-                const int e1 = cell_start[c + 1];
-                for (int e = cell_start[c]; e < e1; ++e) {
-                    const double d1 = x - UCOL(cols, cap, C_X, e);
-                    const double d2 = y - UCOL(cols, cap, C_Y, e);
-                    const double d3 = z - UCOL(cols, cap, C_Z, e);
-                    double s = d1 * d1 + d2 * d2;
-                    s = s + d3 * d3;
-                    const double r = sqrt(s);
-                    if (r > g.h) continue;
-                    const double phi =
-                        fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
-                    const double w = phi * m;
-                    const double k = probe_w(g, r);
-                    a0 = a0 + w * k;
-                    a1 = a1 + (w * UCOL(cols, cap, C_VX, e)) * k;
-                    a2 = a2 + (w * UCOL(cols, cap, C_VY, e)) * k;
-                    a3 = a3 + (w * UCOL(cols, cap, C_VZ, e)) * k;
-                }
+    for (int base = 0; base < total; base += 64) {
+        const int idx = base + lane;
+        bool take = false;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        // The cell of candidate idx: the first t with incl[t] > idx, i.e. the
+        // number of t with incl[t] <= idx, by a fixed five-step search that
+        // every lane runs (the shuffles need the whole wave active).
+        int pos = 0;
+        for (int step = 16; step > 0; step >>= 1)
+            if (__shfl(incl, pos + step - 1) <= idx) pos += step;
+        pos = pos < 31 ? pos : 31;
+        const int e = __shfl(my_e0, pos) + (idx - (__shfl(incl, pos) - __shfl(my_cnt, pos)));
+        if (idx < total) {
+            const double d1 = x - UCOL(cols, cap, C_X, e);
+            const double d2 = y - UCOL(cols, cap, C_Y, e);
+            const double d3 = z - UCOL(cols, cap, C_Z, e);
+            double s = d1 * d1 + d2 * d2;
+            s = s + d3 * d3;
+            const double r = sqrt(s);
+            take = !(r > g.h);
+            if (take) {
+                const double phi =
+                    fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
+                const double w = phi * m;
+                const double k = probe_w(g, r);
+                t0 = w * k;
+                t1 = (w * UCOL(cols, cap, C_VX, e)) * k;
+                t2 = (w * UCOL(cols, cap, C_VY, e)) * k;
+                t3 = (w * UCOL(cols, cap, C_VZ, e)) * k;
             }
-    out[4 * i] = a0;
-    out[4 * i + 1] = a1;
-    out[4 * i + 2] = a2;
-    out[4 * i + 3] = a3;
+        }
+        unsigned long long acc = __ballot(take);
+        while (acc) {
+            const int j = __ffsll((long long)acc) - 1;
+            acc &= acc - 1;
+            a0 = a0 + readlane_f64(t0, j);
+            a1 = a1 + readlane_f64(t1, j);
+            a2 = a2 + readlane_f64(t2, j);
+            a3 = a3 + readlane_f64(t3, j);
+        }
+    }
+    if (lane == 0) {
+        out[4 * i] = a0;
+        out[4 * i + 1] = a1;
+        out[4 * i + 2] = a2;
+        out[4 * i + 3] = a3;
+    }
 }
 
 // ------------------------------------------------------------ field I/O --
@@ -401,7 +451,7 @@ void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long lo
 {
     if (np <= 0) return;
     const SysDesc& d = s->sys_host[sys];
-    k_probe_sum<<<nblk(np), 256, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
+    k_probe_sum<<<(unsigned)((np + 3) / 4), 256, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
                                                  s->cols[s->cur], s->cap, probes_dev, (int)np, m,
                                                  fluid_only, out_dev);
 }

@@ -15,7 +15,7 @@
 module SmoothedParticlesHIP
 
 export ParticleSystem, create_cell_list!, apply!, apply_binary!, apply_unary!,
-       ParticleField, set_params!, energy, neighbour_pairs, step!
+       ParticleField, set_params!, energy, neighbour_pairs, step!, probe_sums
 
 const LIB = joinpath(@__DIR__, "..", "smoothedparticles_hip", "libsph_hip.so")
 
@@ -167,6 +167,19 @@ function energy(sys::ParticleSystem, kind::Integer = 0; kw...)
     check(ccall((:sph_reduce, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{SphParams}, Ref{NTuple{3,Float64}}),
                 sys.handle, kind, params(sys; kw...), out))
     return out[][1]
+end
+
+"""SmoothedParticles.sum(sys, func, x) (src/core.jl:240-258) at probe points
+`xs` for the closures of examples/cavity_flow.jl:166-172: a 4 x np matrix
+whose column i holds sum (phi m) W and the three sums (phi m) v_k W."""
+function probe_sums(sys::ParticleSystem, xs::AbstractVector; fluid_only::Bool = false, kw...)
+    np = length(xs)
+    pts = Float64[c for x in xs for c in (x[1], x[2], x[3])]
+    out = zeros(Float64, 4, np)
+    check(ccall((:sph_probe_sum, LIB), Cint,
+                (Ptr{Cvoid}, Ref{SphParams}, Cint, Ptr{Float64}, Int64, Ptr{Float64}),
+                sys.handle, params(sys; kw...), fluid_only ? 1 : 0, pts, np, out))
+    return out
 end
 
 "nsteps fused device steps (density sum, EOS, internal force, kick, drift)."
