@@ -50,6 +50,7 @@ def lib():
             "orc_create_cell_list": (ctypes.c_int64, [ctypes.c_void_p]),
             "orc_neighbour_pairs": (ctypes.c_int64, [ctypes.c_void_p, _pi, ctypes.c_int64]),
             "orc_neighbour_counts": (ctypes.c_int64, [ctypes.c_void_p, _pi]),
+            "orc_matrix_entries": (ctypes.c_int64, [ctypes.c_void_p, _pi, ctypes.c_int64]),
             "orc_probe_sum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _pd,
                                              ctypes.c_int64, _pd]),
             "orc_brute_pairs": (ctypes.c_int64, [_pd, ctypes.c_int64, ctypes.c_double, _pi,
@@ -170,6 +171,13 @@ class OracleSystem:
         total = lib().orc_neighbour_pairs(self._s, None, 0)
         out = np.empty((max(total, 1), 2), dtype=np.int64)
         lib().orc_neighbour_pairs(self._s, out.ctypes.data_as(_pi), total)
+        return out[:total]
+
+    def matrix_entries(self):
+        """The (p, q) visits of assemble_matrix, p == q included."""
+        total = lib().orc_matrix_entries(self._s, None, 0)
+        out = np.empty((max(total, 1), 2), dtype=np.int64)
+        lib().orc_matrix_entries(self._s, out.ctypes.data_as(_pi), total)
         return out[:total]
 
     def neighbour_counts(self):

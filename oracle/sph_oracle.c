@@ -490,6 +490,31 @@ int64_t orc_neighbour_pairs(const orc_sys* s, int64_t* out, int64_t cap)
     return cnt;
 }
 
+/* The (p, q) entries assemble_matrix pushes (src/core.jl:196-225): the same
+ * cell loop as the pair sweep but without the p == q exclusion (:213-216),
+ * each visit once, p ascending.  Writes at most `cap` entries; returns the
+ * total, or -1 without a current cell list. */
+int64_t orc_matrix_entries(const orc_sys* s, int64_t* out, int64_t cap)
+{
+    if (!s->cells_valid || s->periodic) return -1;
+    int64_t cnt = 0;
+    for (int64_t p = 0; p < s->n; ++p) {
+        const double* xp = s->x + 3 * p;
+        int64_t key = find_key(s, xp);
+        for (int t = 0; t < s->n_diff; ++t) {
+            int64_t nk = key + s->key_diff[t];
+            if (!(1 <= nk && nk <= s->key_max)) continue;
+            for (int64_t e = s->cell_start[nk - 1]; e < s->cell_start[nk]; ++e) {
+                int64_t q = s->cell_idx[e];
+                if (dist3(xp, s->x + 3 * q) > s->h) continue;
+                if (cnt < cap) { out[2 * cnt] = p; out[2 * cnt + 1] = q; }
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
 /* Number of accepted pairs per particle (reference visiting semantics). */
 int64_t orc_neighbour_counts(const orc_sys* s, int64_t* counts)
 {

@@ -34,7 +34,7 @@ __all__ = [
     "internal_force_rho0", "balance_of_mass", "reset_rho", "reset_a", "find_pressure",
     "find_pressure_rho0", "find_pressure_continuity", "accelerate", "move",
     "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
-    "probe_sums",
+    "probe_sums", "assemble_vector", "assemble_matrix",
 ]
 
 FLUID = 0.0
@@ -371,6 +371,47 @@ def probe_sums(sys, x, fluid_only=False, **params):
     check(lib().sph_probe_sum(sys._h, ctypes.byref(p), flags, xs.ctypes.data_as(_dptr),
                               xs.shape[0], out.ctypes.data_as(_dptr)))
     return out
+
+
+def _fields(sys):
+    return {k: sys.get(k) for k in N.FIELDS}
+
+
+def assemble_vector(sys, func):
+    """assemble_vector(sys, func) (src/core.jl:175-182): v_i = func(p_i).
+    `func` is vectorised: it receives a dict of the field arrays (reference
+    order) and returns the n values."""
+    v = np.asarray(func(_fields(sys)), dtype=np.float64).reshape(-1)
+    if v.shape[0] != len(sys):
+        raise ValueError(f"func returned {v.shape[0]} values for {len(sys)} particles")
+    return v
+
+
+def assemble_matrix(sys, func):
+    """assemble_matrix(sys, func) (src/core.jl:196-225): the sparse matrix
+    with A_ij = sum of func(p_i, p_j, r_ij) over the visits of the cell loop,
+    which unlike apply_binary! includes q == p (r = 0).  The device finds the
+    pairs (the current cell list's neighbour pairs); `func` is vectorised: it
+    receives the field dict, the row and column index arrays and the FP64
+    distances r = sqrt((d1*d1 + d2*d2) + d3*d3), and returns the values.
+    Entries visited more than once (aliased small grids) are summed, as
+    SparseArrays.sparse does.  Returns a scipy.sparse CSR matrix."""
+    import scipy.sparse
+
+    n = len(sys)
+    pairs = neighbour_pairs(sys)
+    # A particle meets itself once per key_diff entry equal to 0.
+    self_visits = sum(1 for d in sys.key_geometry()[3] if d == 0)
+    diag = np.repeat(np.arange(n, dtype=np.int64), self_visits)
+    I = np.concatenate([pairs[:, 0], diag])
+    J = np.concatenate([pairs[:, 1], diag])
+    x = sys.get("x")
+    d = x[I] - x[J]
+    r = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    V = np.asarray(func(_fields(sys), I, J, r), dtype=np.float64).reshape(-1)
+    if V.shape[0] != I.shape[0]:
+        raise ValueError("func must return one value per (i, j, r) entry")
+    return scipy.sparse.csr_matrix((V, (I, J)), shape=(n, n))
 
 
 def step(sys, nsteps, scheme="wcsph_se", **params):
