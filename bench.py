@@ -11,7 +11,7 @@ artificial viscosity, symplectic Euler kick and drift.
 
 The same workload in plain FP64 (the drop-in path) is timed beside it, and
 the FP64 CPU oracle (a C port of the reference algorithm, OpenMP over
-particles) is timed on one step of the same workload on the host cores.
+particles) is timed on two steps of the same workload on the host cores.
 
 Beside the headline the same JSON line carries:
   accuracy_vs_fp64  mixed against FP64 on the device after 100 and 1000 steps
@@ -25,6 +25,8 @@ Beside the headline the same JSON line carries:
                     served by continuous batching in the mixed mode: p50/p99
                     completion latency and pair-interactions/s, beside the same
                     requests run one at a time in plain FP64.
+  probe_sums        SmoothedParticles.sum at 65536 seeded points of the fluid's
+                    box (Shepard and velocity sums), per call with host buffers.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
        [--no-fp64] [--no-cpu] [--no-accuracy] [--no-batch] [--no-serve]
