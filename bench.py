@@ -121,16 +121,20 @@ def run_gpu(st, precision, steps, warmup, autotune=True):
 PROBES = 1 << 16
 
 
-def probe_leg(sp, g, st, p, reps=5):
-    """SmoothedParticles.sum at PROBES seeded points in the fluid's bounding
-    box (Shepard denominator and velocity sums, fluid only), timed per call
-    with host buffers: staging, both PCIe copies and the kernel."""
+def probe_points(st):
     import numpy as np
 
     rng = np.random.default_rng(1)
     xf = st.x[st.type == 0.0]
     lo, hi = xf.min(axis=0), xf.max(axis=0)
-    P = lo + rng.random((PROBES, 3)) * (hi - lo)
+    return lo + rng.random((PROBES, 3)) * (hi - lo)
+
+
+def probe_leg(sp, g, st, p, reps=5):
+    """SmoothedParticles.sum at PROBES seeded points in the fluid's bounding
+    box (Shepard denominator and velocity sums, fluid only), timed per call
+    with host buffers: staging, both PCIe copies and the kernel."""
+    P = probe_points(st)
     sp.probe_sums(g, P, fluid_only=True, **p)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -306,7 +310,16 @@ def cpu_baseline(st):
     t0 = time.perf_counter()
     pairs = o.step_wcsph(nsteps, use_rho0=True, **st.params())
     el = time.perf_counter() - t0
+    # probe sums: the oracle's serial loop over the first 4096 of the bench's probes
+    o.create_cell_list()
+    P = probe_points(st)[:4096]
+    t1 = time.perf_counter()
+    o.probe_sum(P, st.params()["m"], True)
+    el_p = time.perf_counter() - t1
     return dict(value=2.0 * pairs / el, unit="pair-interactions/s", cores=O.num_threads(),
+                probe_sums_per_s=len(P) / el_p, probe_cores=1,
+                probe_sample=f"{len(P)} of the probe leg's points after the steps, serial C loop, "
+                             f"{el_p:.2f} s",
                 kind="port",
                 sample=f"{nsteps} full steps (cell list, density, EOS, force, integrate) of the "
                        f"same {len(st.x)}-particle workload by the FP64 C port of the reference "
