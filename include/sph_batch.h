@@ -63,6 +63,11 @@ int sph_batch_build_cells(sph_batch* b);
 /* Neighbour pairs of one system under the current cell lists, as
  * sph_neighbour_pairs. */
 int sph_batch_neighbour_pairs(sph_batch* b, int id, int64_t* pairs, int64_t cap, int64_t* n);
+/* Probe sums of one system under the current cell lists, as sph_probe_sum
+ * with the mass of the system's parameters (SmoothedParticles.sum at points,
+ * src/core.jl:240-258).  Periodic systems: SPH_EUNSUP. */
+int sph_batch_probe_sum(sph_batch* b, int id, int flags, const double* x, int64_t np,
+                        double* out);
 
 /* find_rho0! at t = 0 (tests/test_collision_2d.jl:67-69,123-126) for the
  * systems added since the last call: rho0_p = sum_q m W(h, r_pq) + m W(h, 0).

@@ -1421,6 +1421,15 @@ int sph_batch_reduce(sph_batch* b, int id, int what, double* out)
     return reduce_slot(b, slot, what, out);
 }
 
+int sph_batch_probe_sum(sph_batch* b, int id, int flags, const double* x, int64_t np,
+                        double* out)
+{
+    if (!b || (np > 0 && (!x || !out))) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return probe_sum(b, slot, b->sys_host[slot].ph.m, flags, x, np, out);
+}
+
 int sph_batch_build_cells(sph_batch* b)
 {
     if (!b) return fail(SPH_EINVAL, "null batch");

@@ -539,6 +539,16 @@ class Batch:
     def build_cells(self):
         check(lib().sph_batch_build_cells(self._h))
 
+    def probe_sums(self, sid, x, fluid_only=False):
+        """probe_sums of one system (SmoothedParticles.sum at points,
+        src/core.jl:240-258) with the mass of its own parameters."""
+        xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 3))
+        out = np.zeros((xs.shape[0], 4), dtype=np.float64)
+        flags = N.SPH_PROBE_FLUID_ONLY if fluid_only else 0
+        check(lib().sph_batch_probe_sum(self._h, int(sid), flags, xs.ctypes.data_as(_dptr),
+                                        xs.shape[0], out.ctypes.data_as(_dptr)))
+        return out
+
     def neighbour_pairs(self, sid):
         """Accepted ordered pairs of one system under the current cell lists
         (reference indices of that system), unspecified order."""

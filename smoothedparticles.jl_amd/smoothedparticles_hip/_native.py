@@ -105,6 +105,7 @@ SIGNATURES = {
     "sph_reduce": (_i, [_p, _i, ctypes.POINTER(sph_params), _pd]),
     "sph_neighbour_pairs": (_i, [_p, _pi64, _i64, _pi64]),
     "sph_probe_sum": (_i, [_p, ctypes.POINTER(sph_params), _i, _pd, _i64, _pd]),
+    "sph_batch_probe_sum": (_i, [_p, _i, _i, _pd, _i64, _pd]),
     "sph_step": (_i, [_p, _i, ctypes.POINTER(sph_params), _i, _pi64]),
     "sph_step_pairs": (_i, [_p, _pi64, _pi64]),
     "sph_step_tests": (_i, [_p, _pi64]),

@@ -104,3 +104,33 @@ def test_probe_sums_state_and_errors():
     pt = (ctypes.c_double * 3)(0.5, 0.5, 0.0)
     assert N.lib().sph_probe_sum(g._h, ctypes.byref(p), 2, pt, 1, buf) == N.SPH_EINVAL
     assert N.lib().sph_probe_sum(g._h, ctypes.byref(p), 0, pt, -1, buf) == N.SPH_EINVAL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_probe_sums_equal_alone(precision):
+    """In a batch, each reference system's probe sums (with its own mass)
+    are bit-identical to the oracle's for that system; a periodic system is
+    rejected."""
+    rng = np.random.default_rng(8)
+    b = sp.Batch(precision)
+    cases = []
+    for k, (n, h, m) in enumerate([(2000, 0.05, 0.3), (900, 0.08, 1.7)]):
+        x = rng.random((n, 3)) * [1.0, 0.8, 0.0]
+        v = rng.normal(size=(n, 3))
+        v[:, 2] = 0.0
+        typ = (rng.random(n) < 0.3).astype(float)
+        sid = b.add(x, h, (0, 0, 0), (1.0, 0.8, 0.0), v=v, type=typ, **dict(PHYS, m=m))
+        o = O.OracleSystem(h, np.zeros(3), np.array([1.0, 0.8, 0.0]))
+        o.load(x, v, None, None, typ)
+        assert o.create_cell_list() == n
+        cases.append((sid, o, m, x))
+    xp = rng.random((600, 3)) * [1.0, 1.0, 0.0]
+    pid = b.add(xp, 0.1, (0, 0, 0), (1.0, 1.0, 0.0), periodic=True, **PHYS)
+    b.build_cells()
+    for sid, o, m, x in cases:
+        P = probes(rng, x, 0.05, 2, np.array([1.0, 0.8, 1.0]))
+        got = b.probe_sums(sid, P, fluid_only=True)
+        assert np.array_equal(got, o.probe_sum(P, m, True), equal_nan=True)
+    with pytest.raises(sp.SPHError):
+        b.probe_sums(pid, [[0.5, 0.5, 0.0]])
+    b.close()
