@@ -32,6 +32,12 @@
  *                              energy-drift bound of tests/test_collision_2d.jl
  *   periodic systems ......... numpy minimum-image brute force
  *                              (tests/test_oracle_periodic.py)
+ *   probe sums ............... brute force over all particles, an exact visit
+ *                              count, the density sum at particle positions,
+ *                              closed forms, the NaN-key rule
+ *                              (tests/test_oracle_probe.py)
+ *   assemble_matrix visits ... all-pairs brute force, aliased-grid
+ *                              multiplicities (tests/test_oracle_assemble.py)
  * Parity unpinned at the ulp level: the @fastmath kernels (src/kernels.jl)
  * and Julia's libm log in the energy; those are compared by tolerance.
  */
