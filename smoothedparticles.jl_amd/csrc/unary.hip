@@ -208,11 +208,11 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double* part, int nb
 // examples/cavity_flow.jl:166-172.  One wave per probe: the wave visits the
 // cells in key_diff order (di outermost, src/structs.jl:70-82) and each cell
 // in its stored (descending reference index) order, as one concatenated list
-// taken 64 candidates at a time.
-// The lanes test their candidate and form its four terms in parallel; the
-// accepted terms are then added in lane order, each read with readlane, so
-// every sum is accumulated in the reference's order, in FP64 without
-// contraction: the results are bit-exact.
+// taken 64 candidates at a time.  The lanes test their candidates in
+// parallel and append the accepted ones, in order, to a ring; the terms of
+// 64 ring entries at a time are formed in parallel and added in ring order,
+// each read with readlane, so every sum is accumulated in the reference's
+// order, in FP64 without contraction: the results are bit-exact.
 
 // find_key (src/structs.jl:97-106): -1 when Int64(floor(x/h)) throws, i.e.
 // for a NaN or infinite coordinate or one outside the Int64 range.
@@ -252,20 +252,53 @@ __device__ inline double readlane_f64(double v, int j)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The terms of the accepted candidates in ring slots [done, done + cnt) of
+// one probe, cnt <= 64, added to a0..a3 in slot order.
+__device__ inline void probe_flush(const Geo& g, const double* __restrict__ cols, long long cap,
+                                   double m, int fluid_only, const int* qe, const double* qs,
+                                   int done, int cnt, int lane, double& a0, double& a1,
+                                   double& a2, double& a3)
+{
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (lane < cnt) {
+        const int slot = (done + lane) & 127;
+        const int e = qe[slot];
+        const double r = sqrt(qs[slot]);
+        const double phi = fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
+        const double w = phi * m;
+        const double k = probe_w(g, r);
+        t0 = w * k;
+        t1 = (w * UCOL(cols, cap, C_VX, e)) * k;
+        t2 = (w * UCOL(cols, cap, C_VY, e)) * k;
+        t3 = (w * UCOL(cols, cap, C_VZ, e)) * k;
+    }
+    for (int j = 0; j < cnt; ++j) {
+        a0 = a0 + readlane_f64(t0, j);
+        a1 = a1 + readlane_f64(t1, j);
+        a2 = a2 + readlane_f64(t2, j);
+        a3 = a3 + readlane_f64(t3, j);
+    }
+}
+
 // out[4i] = sum of (phi m) W(h, r), out[4i+1+k] = sum of ((phi m) v_k) W(h, r)
 // over the particles visited from probe i whose r > h is false (a NaN r is
 // kept, as in the reference); phi = Float64(type == FLUID) with fluid_only,
-// else 1.  Block of 4 waves, probe i = 4 * block + wave.
-__global__ __launch_bounds__(256) void k_probe_sum(Geo g, int cell_base,
-                                                   const int* __restrict__ cell_start,
-                                                   const double* __restrict__ cols, long long cap,
-                                                   const double* __restrict__ probes, int np,
-                                                   double m, int fluid_only,
-                                                   double* __restrict__ out)
+// else 1.  One probe per block of one wave.  The test r > h is evaluated as
+// s > s_thr (s_thr = max{s : RN(sqrt(s)) <= h}, and a NaN s passes both);
+// accepted candidates are appended in order to a 128-slot ring, and the
+// square root, W and the terms are formed 64 accepted candidates at a time.
+__global__ __launch_bounds__(64) void k_probe_sum(Geo g, int cell_base,
+                                                  const int* __restrict__ cell_start,
+                                                  const double* __restrict__ cols, long long cap,
+                                                  const double* __restrict__ probes, int np,
+                                                  double m, int fluid_only,
+                                                  double* __restrict__ out)
 {
-    const int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    const int lane = (int)(threadIdx.x & 63);
-    if (i >= np) return;  // wave-uniform
+    __shared__ int qe[128];
+    __shared__ double qs[128];
+    const int i = blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    if (i >= np) return;  // block-uniform
     const double x = probes[3 * i], y = probes[3 * i + 1], z = probes[3 * i + 2];
     const long long key = probe_key(g, x, y, z);
     const long long L1 = g.L1, L12 = (long long)g.L1 * g.L2;
@@ -292,47 +325,45 @@ __global__ __launch_bounds__(256) void k_probe_sum(Geo g, int cell_base,
     }
     const int total = __shfl(incl, 63);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int fill = 0, done = 0;  // ring positions, uniform
+    int t0c = 0;             // first cell holding candidate `base`, uniform
     for (int base = 0; base < total; base += 64) {
         const int idx = base + lane;
+        // The cells covering candidates [base, base + 64), walked with
+        // uniform indices: t0c ends below ncell because base < total.
+        while (__builtin_amdgcn_readlane(incl, t0c) <= base) ++t0c;
+        int e = 0;
+        for (int u = t0c; u < ncell; ++u) {
+            const int hi = __builtin_amdgcn_readlane(incl, u);
+            const int lo = hi - __builtin_amdgcn_readlane(my_cnt, u);
+            if (lo >= base + 64) break;
+            if (idx >= lo && idx < hi) e = __builtin_amdgcn_readlane(my_e0, u) + (idx - lo);
+        }
         bool take = false;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        // The cell of candidate idx: the first t with incl[t] > idx, i.e. the
-        // number of t with incl[t] <= idx, by a fixed five-step search that
-        // every lane runs (the shuffles need the whole wave active).
-        int pos = 0;
-        for (int step = 16; step > 0; step >>= 1)
-            if (__shfl(incl, pos + step - 1) <= idx) pos += step;
-        pos = pos < 31 ? pos : 31;
-        const int e = __shfl(my_e0, pos) + (idx - (__shfl(incl, pos) - __shfl(my_cnt, pos)));
+        double s = 0.0;
         if (idx < total) {
             const double d1 = x - UCOL(cols, cap, C_X, e);
             const double d2 = y - UCOL(cols, cap, C_Y, e);
             const double d3 = z - UCOL(cols, cap, C_Z, e);
-            double s = d1 * d1 + d2 * d2;
+            s = d1 * d1 + d2 * d2;
             s = s + d3 * d3;
-            const double r = sqrt(s);
-            take = !(r > g.h);
-            if (take) {
-                const double phi =
-                    fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
-                const double w = phi * m;
-                const double k = probe_w(g, r);
-                t0 = w * k;
-                t1 = (w * UCOL(cols, cap, C_VX, e)) * k;
-                t2 = (w * UCOL(cols, cap, C_VY, e)) * k;
-                t3 = (w * UCOL(cols, cap, C_VZ, e)) * k;
-            }
+            take = !(s > g.s_thr);  // src/core.jl:252
         }
-        unsigned long long acc = __ballot(take);
-        while (acc) {
-            const int j = __ffsll((long long)acc) - 1;
-            acc &= acc - 1;
-            a0 = a0 + readlane_f64(t0, j);
-            a1 = a1 + readlane_f64(t1, j);
-            a2 = a2 + readlane_f64(t2, j);
-            a3 = a3 + readlane_f64(t3, j);
+        const unsigned long long acc = __ballot(take);
+        if (take) {
+            const int slot = (fill + (int)__popcll(acc & ((1ull << lane) - 1))) & 127;
+            qe[slot] = e;
+            qs[slot] = s;
+        }
+        fill += (int)__popcll(acc);
+        __syncthreads();
+        if (fill - done >= 64) {
+            probe_flush(g, cols, cap, m, fluid_only, qe, qs, done, 64, lane, a0, a1, a2, a3);
+            done += 64;
+            __syncthreads();  // the slots read here are rewritten by the next appends
         }
     }
+    probe_flush(g, cols, cap, m, fluid_only, qe, qs, done, fill - done, lane, a0, a1, a2, a3);
     if (lane == 0) {
         out[4 * i] = a0;
         out[4 * i + 1] = a1;
@@ -451,7 +482,7 @@ void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long lo
 {
     if (np <= 0) return;
     const SysDesc& d = s->sys_host[sys];
-    k_probe_sum<<<(unsigned)((np + 3) / 4), 256, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
+    k_probe_sum<<<(unsigned)np, 64, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
                                                  s->cols[s->cur], s->cap, probes_dev, (int)np, m,
                                                  fluid_only, out_dev);
 }
