@@ -134,3 +134,22 @@ def test_batch_probe_sums_equal_alone(precision):
     with pytest.raises(sp.SPHError):
         b.probe_sums(pid, [[0.5, 0.5, 0.0]])
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_probe_sums_aliased_grid(precision):
+    """A grid one cell wide: three key_diff entries reach the probe's own
+    cell, and rows wrap; the reference then visits cells several times and
+    so must the device, in the same order."""
+    rng = np.random.default_rng(12)
+    n, h = 400, 0.1
+    x = rng.random((n, 3)) * [0.05, 1.0, 0.0]
+    v = rng.normal(size=(n, 3))
+    typ = (rng.random(n) < 0.3).astype(float)
+    hi = np.array([0.05, 1.0, 0.0])
+    g, o = make_both(x, h, np.zeros(3), hi, precision, v, typ)
+    assert sp.create_cell_list(g) == o.create_cell_list() == n
+    P = rng.random((200, 3)) * [0.06, 1.0, 0.0]
+    P[:20] = x[:20]
+    got = sp.probe_sums(g, P, **PHYS)
+    assert np.array_equal(got, o.probe_sum(P, PHYS["m"], False), equal_nan=True)
