@@ -61,6 +61,9 @@ typedef struct sph_system sph_system;
 #define SPH_FIELD_DRHO 5  /* density rate [n] */
 #define SPH_FIELD_RHO0 6  /* per-particle reference density [n] */
 #define SPH_FIELD_TYPE 7  /* particle type, FLUID = 0.0 [n] */
+#define SPH_FIELD_KEY 8   /* read only: find_key of the particle at the last cell-list
+                             build (1-based, src/structs.jl:97-106), as a double [n];
+                             periodic systems: their own cell key */
 
 /* Binary operators, registered by the name of the reference operator they
  * replace.  Each runs over the neighbour set of src/core.jl:94-112. */
@@ -83,8 +86,12 @@ typedef struct sph_system sph_system;
 #define SPH_OP_FIND_PRESSURE_CONTINUITY 20 /* find_pressure! examples/collapse_dry.jl:123-127 */
 #define SPH_OP_ACCELERATE 21         /* accelerate! fluid: v += (coeff*dt)(a + g)
                                         examples/collapse_dry.jl:155-159 */
-#define SPH_OP_MOVE 22               /* move! fluid: x += (coeff*dt) v
-                                        examples/collapse_dry.jl:148-153 */
+#define SPH_OP_MOVE 22               /* move! examples/collapse_dry.jl:148-153: a = 0 for
+                                        every particle, then fluid: x += (coeff*dt) v */
+#define SPH_OP_DRIFT 23              /* move! tests/test_collision_2d.jl:88-90: every
+                                        particle x += (coeff*dt) v; a is kept */
+#define SPH_OP_KICK 24               /* accelerate! tests/test_collision_2d.jl:92-94:
+                                        every particle v += (coeff*dt) a (no gravity) */
 
 /* Reductions (sph_reduce) */
 #define SPH_REDUCE_ENERGY 0           /* sum of energy(p), examples/collapse_dry.jl:166-171 (1 value) */
@@ -100,6 +107,16 @@ typedef struct sph_system sph_system;
 #define SPH_SCHEME_WCSPH_SE_RHO0 1  /* the same with P = c^2 (rho - rho0_p), rho0_p the
                                        per-particle field set by find_rho0! at t = 0
                                        (tests/test_collision_2d.jl:67-73,123-126) */
+#define SPH_SCHEME_COLLAPSE_DRY 2   /* the time loop of examples/collapse_dry.jl:202-211:
+                                       accelerate!, move!, create_cell_list!,
+                                       balance_of_mass!, find_pressure! (continuity),
+                                       move!, create_cell_list!, internal_force!,
+                                       accelerate!, with the half steps 0.5*dt of its
+                                       move!/accelerate! (:148-159); params->coeff is
+                                       not used.  The caller builds the cells and runs
+                                       internal_force! once before the first step
+                                       (:198-200).  The pair counts of sph_step_pairs
+                                       are those of balance_of_mass! and internal_force! */
 
 typedef struct sph_params {
     double m;      /* particle mass */
@@ -172,6 +189,18 @@ int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const doub
  * (p, q), p != q, dist <= h: pairs[2k], pairs[2k+1].  The order is
  * unspecified.  *n receives the total; at most cap pairs are written. */
 int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n);
+
+/* Diagnostics (no counterpart in the reference): an order-independent
+ * digest of every particle's neighbour set under the current cell list, for
+ * comparing the sets of _apply_binary! (src/core.jl:94-112) at sizes where
+ * listing the pairs is too large.  For particle p (reference order, n must
+ * equal sph_count) counts[p] receives the number of accepted q and
+ * hashes[p] the sum modulo 2^64 of mix64(q + 1) over them, mix64 being the
+ * splitmix64 finaliser:
+ *   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9; z = (z ^ (z >> 27)) *
+ *   0x94d049bb133111eb; return z ^ (z >> 31).
+ * Either output may be NULL. */
+int sph_neighbour_digest(sph_system* s, int64_t* counts, uint64_t* hashes, int64_t n);
 
 /* nsteps fused steps of a scheme, device-resident, one host sync at the end.
  * *pairs (may be NULL) receives the accepted ordered pairs summed over the

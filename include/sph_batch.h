@@ -63,6 +63,10 @@ int sph_batch_build_cells(sph_batch* b);
 /* Neighbour pairs of one system under the current cell lists, as
  * sph_neighbour_pairs. */
 int sph_batch_neighbour_pairs(sph_batch* b, int id, int64_t* pairs, int64_t cap, int64_t* n);
+/* Neighbour-set digest of one system under the current cell lists, as
+ * sph_neighbour_digest (n must equal the system's particle count). */
+int sph_batch_neighbour_digest(sph_batch* b, int id, int64_t* counts, uint64_t* hashes,
+                               int64_t n);
 /* Probe sums of one system under the current cell lists, as sph_probe_sum
  * with the mass of the system's parameters (SmoothedParticles.sum at points,
  * src/core.jl:240-258).  Periodic systems: SPH_EUNSUP. */

@@ -59,13 +59,19 @@ def lib():
             "orc_density_sum_rho0": (None, [ctypes.c_void_p, _pp, ctypes.c_int]),
             "orc_internal_force": (None, [ctypes.c_void_p, _pp]),
             "orc_pressure_force_rho0": (None, [ctypes.c_void_p, _pp]),
-            "orc_balance_of_mass": (None, [ctypes.c_void_p, _pp]),
+            "orc_balance_of_mass": (ctypes.c_int64, [ctypes.c_void_p, _pp]),
             "orc_reset_rho": (None, [ctypes.c_void_p]),
             "orc_reset_a": (None, [ctypes.c_void_p]),
             "orc_eos": (None, [ctypes.c_void_p, _pp, ctypes.c_int]),
             "orc_eos_continuity": (None, [ctypes.c_void_p, _pp]),
             "orc_accelerate": (None, [ctypes.c_void_p, _pp]),
             "orc_move": (None, [ctypes.c_void_p, _pp]),
+            "orc_drift": (None, [ctypes.c_void_p, _pp]),
+            "orc_kick": (None, [ctypes.c_void_p, _pp]),
+            "orc_step_collapse_dry": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int]),
+            "orc_neighbour_digest": (ctypes.c_int64, [ctypes.c_void_p, _pi,
+                                                      ctypes.POINTER(ctypes.c_uint64)]),
+            "orc_mix64": (ctypes.c_uint64, [ctypes.c_uint64]),
             "orc_energy": (ctypes.c_double, [ctypes.c_void_p, _pp, ctypes.c_int]),
             "orc_momentum": (None, [ctypes.c_void_p, _pp, _pd]),
             "orc_step_wcsph": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int, ctypes.c_int]),
@@ -214,7 +220,7 @@ class OracleSystem:
 
     def balance_of_mass(self, **kw):
         p = params(**kw)
-        lib().orc_balance_of_mass(self._s, ctypes.byref(p))
+        return lib().orc_balance_of_mass(self._s, ctypes.byref(p))
 
     def reset_rho(self):
         lib().orc_reset_rho(self._s)
@@ -238,6 +244,32 @@ class OracleSystem:
         p = params(**kw)
         lib().orc_move(self._s, ctypes.byref(p))
 
+    def drift(self, **kw):
+        p = params(**kw)
+        lib().orc_drift(self._s, ctypes.byref(p))
+
+    def kick(self, **kw):
+        p = params(**kw)
+        lib().orc_kick(self._s, ctypes.byref(p))
+
+    def step_collapse_dry(self, nsteps, **kw):
+        """nsteps iterations of the time loop of examples/collapse_dry.jl:202-211;
+        returns the balance_of_mass! pairs summed over them."""
+        p = params(**kw)
+        return lib().orc_step_collapse_dry(self._s, ctypes.byref(p), int(nsteps))
+
+    def neighbour_digest(self):
+        """(counts, hashes): per particle the number of neighbours and the sum
+        modulo 2^64 of mix64(q + 1) over them."""
+        n = self.n
+        c = np.empty(max(n, 1), dtype=np.int64)
+        hsh = np.empty(max(n, 1), dtype=np.uint64)
+        tot = lib().orc_neighbour_digest(self._s, c.ctypes.data_as(_pi),
+                                         hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if tot < 0:
+            raise RuntimeError("no current cell list")
+        return c[:n], hsh[:n]
+
     def energy(self, kind="wcsph", **kw):
         p = params(**kw)
         k = {"wcsph": 0, "collision": 1, "wcsph_rho0": 2}[kind]
@@ -260,6 +292,10 @@ def brute_pairs(x, h):
     out = np.empty((max(total, 1), 2), dtype=np.int64)
     lib().orc_brute_pairs(x.ctypes.data_as(_pd), len(x), float(h), out.ctypes.data_as(_pi), total)
     return out[:total]
+
+
+def mix64(z):
+    return lib().orc_mix64(int(z))
 
 
 def num_threads():

@@ -536,6 +536,36 @@ int64_t orc_neighbour_counts(const orc_sys* s, int64_t* counts)
     return tot;
 }
 
+/* Order-independent digest of each particle's neighbour set, for comparing
+ * sets at sizes where listing the pairs is too large: counts[p] is the
+ * number of accepted q, hash[p] the sum modulo 2^64 of mix(q + 1) over them,
+ * mix being the splitmix64 finaliser (Steele, Lea and Flood 2014, constants
+ * as published by Vigna).  A repeated visit (aliased small grids) counts
+ * twice, so equal digests mean equal multisets with overwhelming
+ * probability.  Same neighbour loop as _apply_binary! (src/core.jl:94-112). */
+uint64_t orc_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+int64_t orc_neighbour_digest(const orc_sys* s, int64_t* counts, uint64_t* hash)
+{
+    if (!s->cells_valid) return -1;
+    int64_t tot = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : tot)
+    for (int64_t p = 0; p < s->n; ++p) {
+        int64_t c = 0;
+        uint64_t hh = 0;
+        FOR_EACH_NEIGHBOUR(s, p, { (void)r; (void)xq; ++c; hh += orc_mix64((uint64_t)q + 1); });
+        counts[p] = c;
+        hash[p] = hh;
+        tot += c;
+    }
+    return tot;
+}
+
 /* SmoothedParticles.sum(sys, func, x), src/core.jl:240-258, at np probe
  * points x[np][3], with the interpolation closures of
  * examples/cavity_flow.jl:166-172: out[4i] = sum of (phi*m)*W(h,r) and
@@ -686,10 +716,11 @@ void orc_pressure_force_rho0(orc_sys* s, const orc_params* pr)
 
 /* examples/collapse_dry.jl:112-115 (balance_of_mass!): continuity equation
  * with density diffusion nu. */
-void orc_balance_of_mass(orc_sys* s, const orc_params* pr)
+int64_t orc_balance_of_mass(orc_sys* s, const orc_params* pr)
 {
     double m = pr->m, nu = pr->nu;
-#pragma omp parallel for schedule(dynamic, 256)
+    int64_t cnt = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : cnt)
     for (int64_t p = 0; p < s->n; ++p) {
         const double* xp = s->x + 3 * p;
         const double* vp = s->v + 3 * p;
@@ -701,9 +732,11 @@ void orc_balance_of_mass(orc_sys* s, const orc_params* pr)
                        (xp[1] - xq[1]) * (vp[1] - vq[1]);
             d = d + (xp[2] - xq[2]) * (vp[2] - vq[2]);
             acc = acc + ker * (d + 2.0 * nu * (s->rho[p] - s->rho[q]));
+            ++cnt;
         });
         s->Drho[p] = acc;
     }
+    return cnt;
 }
 
 /* ---------------------------------------------------------- unary ops -- */
@@ -746,8 +779,43 @@ void orc_accelerate(orc_sys* s, const orc_params* pr)
     }
 }
 
-/* examples/collapse_dry.jl:148-153 (move!): fluid only, x += (coeff*dt)*v. */
+/* examples/collapse_dry.jl:148-153 (move!): p.Dv = VEC0 for every
+ * particle, then fluid only x += (coeff*dt)*v. */
 void orc_move(orc_sys* s, const orc_params* pr)
+{
+    double k = pr->coeff * pr->dt;
+    for (int64_t i = 0; i < s->n; ++i) {
+        s->a[3 * i] = 0.0; s->a[3 * i + 1] = 0.0; s->a[3 * i + 2] = 0.0;
+        if (s->type[i] != 0.0) continue;
+        for (int d = 0; d < 3; ++d)
+            s->x[3 * i + d] = s->x[3 * i + d] + k * s->v[3 * i + d];
+    }
+}
+
+/* tests/test_collision_2d.jl:88-90 (move!): every particle x += (coeff*dt)*v;
+ * the acceleration is kept. */
+void orc_drift(orc_sys* s, const orc_params* pr)
+{
+    double k = pr->coeff * pr->dt;
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int d = 0; d < 3; ++d)
+            s->x[3 * i + d] = s->x[3 * i + d] + k * s->v[3 * i + d];
+}
+
+/* tests/test_collision_2d.jl:92-94 (accelerate!): every particle
+ * v += (coeff*dt)*a, no gravity. */
+void orc_kick(orc_sys* s, const orc_params* pr)
+{
+    double k = pr->coeff * pr->dt;
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int d = 0; d < 3; ++d)
+            s->v[3 * i + d] = s->v[3 * i + d] + k * s->a[3 * i + d];
+}
+
+/* The drift of the fused symplectic Euler step: the fluid drift of
+ * examples/collapse_dry.jl:150-152 without its reset of Dv, so the step
+ * leaves the last force in a. */
+static void drift_fluid(orc_sys* s, const orc_params* pr)
 {
     double k = pr->coeff * pr->dt;
     for (int64_t i = 0; i < s->n; ++i) {
@@ -863,7 +931,34 @@ int64_t orc_step_wcsph(orc_sys* s, const orc_params* pr, int nsteps, int use_rho
         orc_reset_a(s);
         orc_internal_force(s, &p1);
         orc_accelerate(s, &p1);
-        orc_move(s, &p1);
+        drift_fluid(s, &p1);
+    }
+    return pairs;
+}
+
+/* The time loop of examples/collapse_dry.jl:202-211, nsteps iterations:
+ *   accelerate!; move!; create_cell_list!; balance_of_mass!;
+ *   find_pressure! (continuity form); move!; create_cell_list!;
+ *   internal_force!; accelerate!
+ * with the half steps 0.5*dt of accelerate! and move! (:148-159).  The
+ * caller runs the initial create_cell_list! and internal_force! (:198-200).
+ * Returns the accepted ordered pairs of the balance_of_mass! passes summed
+ * over the steps. */
+int64_t orc_step_collapse_dry(orc_sys* s, const orc_params* pr, int nsteps)
+{
+    orc_params ph = *pr;
+    ph.coeff = 0.5;
+    int64_t pairs = 0;
+    for (int k = 0; k < nsteps; ++k) {
+        orc_accelerate(s, &ph);
+        orc_move(s, &ph);
+        orc_create_cell_list(s);
+        pairs += orc_balance_of_mass(s, &ph);
+        orc_eos_continuity(s, &ph);
+        orc_move(s, &ph);
+        orc_create_cell_list(s);
+        orc_internal_force(s, &ph);
+        orc_accelerate(s, &ph);
     }
     return pairs;
 }

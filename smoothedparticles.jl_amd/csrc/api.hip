@@ -22,11 +22,13 @@ void enqueue_tiles(sph_system* s);
 void enqueue_unary(sph_system* s, int op, double* prho2);
 void enqueue_prho2(sph_system* s);
 void enqueue_kick_drift(sph_system* s);
+void enqueue_cd_unary(sph_system* s, int stage);
 void enqueue_reduce(sph_system* s, int sys, long long off, long long n, int what, double* out_dev);
 void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long long np, double m,
                        int fluid_only, double* out_dev);
 void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,
                        double* out_dev);
+void enqueue_key_out(sph_system* s, long long off, long long n, int cell_base, double* out_dev);
 void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,
                       const double* in_dev);
 void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const double* in_dev,
@@ -493,7 +495,7 @@ int run_pair(sph_system* s, int op, int self, int overwrite, unsigned long long*
 }
 
 bool is_binary(int op) { return op >= SPH_OP_DENSITY_SUM && op <= SPH_OP_BALANCE_OF_MASS; }
-bool is_unary(int op) { return op >= SPH_OP_RESET_RHO && op <= SPH_OP_MOVE; }
+bool is_unary(int op) { return op >= SPH_OP_RESET_RHO && op <= SPH_OP_KICK; }
 
 int col_of_field(int field, int* ncomp)
 {
@@ -705,9 +707,45 @@ int set_params0(sph_system* s, const sph_params* p)
     return 0;
 }
 
+// The time loop of examples/collapse_dry.jl:202-211 (see k_cd_unary):
+// accelerate! + move! (half steps); build; balance_of_mass!; continuity
+// find_pressure!; move!; build; P/rho^2 in the new order; internal_force!;
+// accelerate!.
+int cd_step(sph_system* s, long long tiles_bound)
+{
+    int rc;
+    hipEvent_t ev[2];
+    prof_begin(s, ev);
+    enqueue_cd_unary(s, CD_KICK_MOVE);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
+    prof_begin(s, ev);
+    enqueue_build(s);
+    prof_end(s, SPH_KERNEL_BUILD, ev);
+    if ((rc = run_pair(s, P_CONTINUITY, 0, 0, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
+                       MASK_TEST, 0, s->pair_counter + 2)))
+        return rc;
+    prof_begin(s, ev);
+    enqueue_unary(s, SPH_OP_FIND_PRESSURE_CONTINUITY, nullptr);
+    enqueue_cd_unary(s, CD_MOVE);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
+    prof_begin(s, ev);
+    enqueue_build(s);
+    prof_end(s, SPH_KERNEL_BUILD, ev);
+    enqueue_prho2(s);  // the build permuted the particles
+    if ((rc = run_pair(s, P_FORCE, 0, 0, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE,
+                       MASK_TEST)))
+        return rc;
+    prof_begin(s, ev);
+    enqueue_cd_unary(s, CD_KICK);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
 int run_steps(sph_system* s, int scheme, int nsteps)
 {
-    if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0)
+    if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0 &&
+        scheme != SPH_SCHEME_COLLAPSE_DRY)
         return fail(SPH_EUNSUP, "unknown scheme");
     if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
     const int eos = scheme == SPH_SCHEME_WCSPH_SE ? SPH_OP_FIND_PRESSURE : SPH_OP_FIND_PRESSURE_RHO0;
@@ -720,6 +758,10 @@ int run_steps(sph_system* s, int scheme, int nsteps)
     const long long tiles_bound = s->tile_cap;
     HIPC(hipMemsetAsync(s->pair_counter, 0, 3 * sizeof(unsigned long long), s->stream));
     for (int k = 0; k < nsteps; ++k) {
+        if (scheme == SPH_SCHEME_COLLAPSE_DRY) {
+            if ((rc = cd_step(s, tiles_bound))) return rc;
+            continue;
+        }
         hipEvent_t ev[2];
         prof_begin(s, ev);
         enqueue_build(s);
@@ -831,6 +873,17 @@ int compact(sph_system* s)
 
 int field_io(sph_system* s, int slot, int field, double* host, int64_t n, bool in)
 {
+    if (field == SPH_FIELD_KEY) {  // read only, from the last build
+        if (in) return fail(SPH_EINVAL, "the cell-key field is read only");
+        if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+        if (n != s->sys_n[slot]) return fail(SPH_EINVAL, "field length does not match the particle count");
+        if (n == 0) return 0;
+        enqueue_key_out(s, s->sys_off[slot], n, s->sys_host[slot].cell_base, s->stage);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(host, s->stage, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIPC(hipStreamSynchronize(s->stream));
+        return 0;
+    }
     int nc;
     const int col = col_of_field(field, &nc);
     if (col < 0) return fail(SPH_EINVAL, "unknown field");
@@ -946,6 +999,59 @@ int neighbour_pairs(sph_system* s, int only_slot, int64_t* pairs, int64_t cap, i
             if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
             dfree(s, dpairs, 2 * ncopy);
             if (e != hipSuccess) rc = fail(SPH_EHIP, std::string("pair emission: ") + hipGetErrorString(e));
+        }
+    }
+    if (flagged) {
+        for (int i = 0; i < nsys(s); ++i) s->sys_host[i].pass_on = 0;
+        const std::string msg = g_err;
+        const int rc2 = upload_sysd(s);
+        if (rc) g_err = msg;
+        else rc = rc2;
+    }
+    return rc;
+}
+
+// Per-particle neighbour count and hash of the current cell lists (of
+// system `slot`), in that system's reference order.
+int neighbour_digest(sph_system* s, int slot, int64_t* counts, uint64_t* hashes, int64_t n)
+{
+    if ((long long)n != s->sys_n[slot]) return fail(SPH_EINVAL, "n must equal the particle count");
+    if (n == 0) return 0;
+    int rc;
+    const int flagged = nsys(s) > 1;
+    if (flagged) {
+        for (int i = 0; i < nsys(s); ++i) s->sys_host[i].pass_on = i == slot;
+        if ((rc = upload_sysd(s))) return rc;
+    }
+    long long* dout = nullptr;
+    rc = dalloc(s, &dout, 2 * (long long)n);
+    if (!rc) {
+        PairCtx c = make_ctx(s);
+        c.pair_out = dout;
+        c.pair_cap = 2 * (long long)n;
+        c.only_flagged = flagged;
+        hipError_t e = hipMemsetAsync(dout, 0, sizeof(long long) * 2 * n, s->stream);
+        if (e == hipSuccess) {
+            reset_queue(s);
+            launch_pair(P_DIGEST, s->precision, s->rows, s->flat, s->periodic, c,
+                        pair_grid(s->meta_host[META_NTILES]), s->stream);
+            e = hipGetLastError();
+        }
+        std::vector<long long> host;
+        if (e == hipSuccess) {
+            host.resize(2 * (size_t)n);
+            e = hipMemcpyAsync(host.data(), dout, sizeof(long long) * 2 * n,
+                               hipMemcpyDeviceToHost, s->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        dfree(s, dout, 2 * (long long)n);
+        if (e != hipSuccess) {
+            rc = fail(SPH_EHIP, std::string("neighbour digest: ") + hipGetErrorString(e));
+        } else {
+            for (int64_t i = 0; i < n; ++i) {
+                if (counts) counts[i] = host[2 * i];
+                if (hashes) hashes[i] = (uint64_t)host[2 * i + 1];
+            }
         }
     }
     if (flagged) {
@@ -1146,7 +1252,7 @@ int sph_apply_unary(sph_system* s, int op, const sph_params* params)
     prof_end(s, SPH_KERNEL_UNARY, ev);
     HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(s->stream));
-    if (op == SPH_OP_MOVE) s->cells_valid = s->masks_valid = false;
+    if (op == SPH_OP_MOVE || op == SPH_OP_DRIFT) s->cells_valid = s->masks_valid = false;
     return 0;
 }
 
@@ -1178,6 +1284,14 @@ int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
     if (!s || !n) return fail(SPH_EINVAL, "null argument");
     if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
     return neighbour_pairs(s, -1, pairs, cap, n);
+}
+
+int sph_neighbour_digest(sph_system* s, int64_t* counts, uint64_t* hashes, int64_t n)
+{
+    if (!s || (n > 0 && !counts && !hashes)) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    return neighbour_digest(s, 0, counts, hashes, n);
 }
 
 int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps, int64_t* pairs)
@@ -1445,6 +1559,16 @@ int sph_batch_neighbour_pairs(sph_batch* b, int id, int64_t* pairs, int64_t cap,
     const int slot = slot_of(b, id);
     if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
     return neighbour_pairs(b, slot, pairs, cap, n);
+}
+
+int sph_batch_neighbour_digest(sph_batch* b, int id, int64_t* counts, uint64_t* hashes,
+                               int64_t n)
+{
+    if (!b || (n > 0 && !counts && !hashes)) return fail(SPH_EINVAL, "null argument");
+    if (!b->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return neighbour_digest(b, slot, counts, hashes, n);
 }
 
 int sph_batch_init_rho0(sph_batch* b)

@@ -874,6 +874,77 @@ struct Pairs64 {
                                   bool) {}
 };
 
+// Neighbour-set digest of each target: the number of accepted q and the sum
+// modulo 2^64 of mix64(refidx q + 1), written to pair_out[2 refidx p] and
+// pair_out[2 refidx p + 1].  Integer sums, so the result does not depend on
+// the order in which the pairs are met.
+__device__ inline unsigned long long mix64(unsigned long long z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__device__ inline void digest_write(const PairCtx& c, int p, unsigned long long h,
+                                    unsigned long long n, int T, bool write)
+{
+    for (int o = T; o < 64; o <<= 1) {
+        h += __shfl_xor(h, o);
+        n += __shfl_xor(n, o);
+    }
+    if (!write) return;
+    const long long r = c.refidx[p];
+    c.pair_out[2 * r] = (long long)n;
+    c.pair_out[2 * r + 1] = (long long)h;
+}
+
+struct Digest64 {
+    static constexpr bool INTERACT = true;
+    static constexpr int WAVES = 5;
+    using Lds = Lds64Pos;
+    struct Tgt {
+        double x, y, z;
+    };
+    struct Acc {
+        unsigned long long h, n;
+    };
+    static __device__ void load(const PairCtx& c, int p, Tgt& t)
+    {
+        t.x = COL(c, C_X, p); t.y = COL(c, C_Y, p); t.z = COL(c, C_Z, p);
+    }
+    static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
+    {
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    static __device__ void put(const Raw& r, int l, Lds& s)
+    {
+        s.xy[l] = make_double2(r.x, r.y);
+        s.z[l] = r.z;
+    }
+    static __device__ void init(Acc& a) { a.h = a.n = 0; }
+    using Part = Acc;
+    static __device__ void zero(Part& a) { a.h = a.n = 0; }
+    static __device__ void fold(Acc& a, const Part& q) { a.h += q.h; a.n += q.n; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+                                    int, int, int q, double, double, double, double, Acc& a)
+    {
+        a.h += mix64((unsigned long long)c.refidx[q] + 1ull);
+        a.n += 1;
+    }
+    static __device__ void finish(const PairCtx& c, const SysDesc&, int p, const Tgt&, Acc& a,
+                                  int T, bool write)
+    {
+        digest_write(c, p, a.h, a.n, T, write);
+    }
+};
+
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
 template <int V, bool FLAT, int T, class Op>
 __device__ inline void walk64(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
@@ -1126,7 +1197,8 @@ struct Density32 {
     struct Acc {
         float s, e;
     };
-    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+                                double oz, Tgt& t)
     {
         t.x = (float)(COL(c, C_X, p) - ox);
         t.y = (float)(COL(c, C_Y, p) - oy);
@@ -1141,7 +1213,8 @@ struct Density32 {
         return r;
     }
     template <bool FLAT>
-    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+                                    double oz, Lds& s)
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
@@ -1188,7 +1261,8 @@ struct Force32 {
     struct Acc {
         float sx, sy, sz, ex, ey, ez, lx, ly, lz, fx, fy, fz;
     };
-    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+                                double oz, Tgt& t)
     {
         t.x = (float)(COL(c, C_X, p) - ox);
         t.y = (float)(COL(c, C_Y, p) - oy);
@@ -1210,7 +1284,8 @@ struct Force32 {
         return r;
     }
     template <bool FLAT>
-    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+                                    double oz, Lds& s)
     {
         float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
         a.w = (float)r.a4;
@@ -1287,6 +1362,159 @@ struct Force32 {
     }
 };
 
+// internal_force! of tests/test_collision_2d.jl:75-78 in the mixed form:
+// a += -kdW/rho0^2 u^3 (P_p + P_q) x_pq for every target, P staged in FP32.
+struct ForceRho032 {
+    static constexpr bool INTERACT = true;
+    using Lds = Lds32Pos;
+    static constexpr bool STAGE_Q2 = false;  // pos.w = P
+    static constexpr int WAVES = 6;
+    struct Tgt {
+        float x, y, z, P;
+    };
+    struct Part {
+        float ax, ay, az;
+    };
+    struct Acc {
+        float sx, sy, sz, ex, ey, ez;
+    };
+    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+                                double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+        t.P = (float)COL(c, C_P, p);
+    }
+    static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
+    {
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_P, q);
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+                                    double oz, Lds& s)
+    {
+        float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
+        a.w = (float)r.a0;
+        s.pos[l] = a;
+    }
+    static __device__ void init(Acc& a) { a.sx = a.sy = a.sz = a.ex = a.ey = a.ez = 0.0f; }
+    static __device__ void zero(Part& q) { q.ax = q.ay = q.az = 0.0f; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds& s, int jj, int, int, float dx, float dy,
+                                    float dz, float ss, Part& q)
+    {
+        const float r = sqrt_hw(ss);
+        const float u = 1.0f - r * (float)d.g.inv_h;
+        const float f = u * u * u * (t.P + s.pos[jj].w);
+        q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
+    }
+    static __device__ void fold(Acc& a, const Part& q)
+    {
+        two_sum_add(a.sx, a.ex, q.ax);
+        two_sum_add(a.sy, a.ey, q.ay);
+        two_sum_add(a.sz, a.ez, q.az);
+    }
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
+                                  const Acc& a, int T, bool write, int)
+    {
+        double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
+               az = (double)a.sz + (double)a.ez;
+        for (int o = T; o < 64; o <<= 1) {
+            ax += __shfl_xor(ax, o); ay += __shfl_xor(ay, o); az += __shfl_xor(az, o);
+        }
+        if (!write) return;
+        const double k = -d.ph.kdW / (d.ph.rho0 * d.ph.rho0);
+        const double bx = c.overwrite ? 0.0 : COL(c, C_AX, p);
+        const double by = c.overwrite ? 0.0 : COL(c, C_AY, p);
+        const double bz = c.overwrite ? 0.0 : COL(c, C_AZ, p);
+        COL(c, C_AX, p) = bx + k * ax;
+        COL(c, C_AY, p) = by + k * ay;
+        COL(c, C_AZ, p) = bz + k * az;
+    }
+};
+
+// balance_of_mass! (examples/collapse_dry.jl:112-115) in the mixed form:
+// Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q)).  The densities are
+// staged as rho - rho0 in FP32, so their difference keeps its precision.
+struct Continuity32 {
+    static constexpr bool INTERACT = true;
+    using Lds = Lds32Full;
+    static constexpr bool STAGE_Q2 = false;  // pos.w = rho - rho0
+    static constexpr int WAVES = 5;
+    struct Tgt {
+        float x, y, z, vx, vy, vz, dr;
+    };
+    struct Part {
+        float d;
+    };
+    struct Acc {
+        float s, e;
+    };
+    static __device__ void load(const PairCtx& c, const SysDesc& d, int p, double ox, double oy,
+                                double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+        t.vx = (float)COL(c, C_VX, p); t.vy = (float)COL(c, C_VY, p); t.vz = (float)COL(c, C_VZ, p);
+        t.dr = (float)(COL(c, C_RHO, p) - d.ph.rho0);
+    }
+    static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
+    {
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        r.a0 = COL(c, C_RHO, q);
+        r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, const SysDesc& d, double ox, double oy,
+                               double oz, Lds& s)
+    {
+        float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
+        a.w = (float)(r.a0 - d.ph.rho0);
+        s.pos[l] = a;
+        s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, 0.0f);
+    }
+    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
+    static __device__ void zero(Part& q) { q.d = 0.0f; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
+                                    const Lds& s, int jj, int, int, float dx, float dy,
+                                    float dz, float ss, Part& q)
+    {
+        const float r = sqrt_hw(ss);
+        const float u = 1.0f - r * (float)d.g.inv_h;
+        const float4 v = s.v[jj];
+        const float xv = fmaf(dz, t.vz - v.z, fmaf(dy, t.vy - v.y, dx * (t.vx - v.x)));
+        const float dd = fmaf(2.0f * (float)d.ph.nu, t.dr - s.pos[jj].w, xv);
+        q.d = fmaf(u * u * u, dd, q.d);
+    }
+    static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.d); }
+    static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
+                                  const Acc& a, int T, bool write, int)
+    {
+        double v = (double)a.s + (double)a.e;
+        for (int o = T; o < 64; o <<= 1) v += __shfl_xor(v, o);
+        if (!write) return;
+        const double b = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
+        COL(c, C_DRHO, p) = b + d.ph.kdW * v;
+    }
+};
+
 template <bool EMIT>
 struct Pairs32 {
     static constexpr bool INTERACT = EMIT;
@@ -1302,7 +1530,8 @@ struct Pairs32 {
     struct Acc {
         int dummy;
     };
-    static __device__ void load(const PairCtx& c, int p, double ox, double oy, double oz, Tgt& t)
+    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+                                double oz, Tgt& t)
     {
         t.x = (float)(COL(c, C_X, p) - ox);
         t.y = (float)(COL(c, C_Y, p) - oy);
@@ -1317,7 +1546,8 @@ struct Pairs32 {
         return r;
     }
     template <bool FLAT>
-    static __device__ void put(const Raw& r, int l, double ox, double oy, double oz, Lds& s)
+    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+                                    double oz, Lds& s)
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
@@ -1338,6 +1568,58 @@ struct Pairs32 {
     static __device__ void fold(Acc&, const Part&) {}
     static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, const Acc&,
                                   int, bool, int) {}
+};
+
+struct Digest32 {
+    static constexpr bool INTERACT = true;
+    using Lds = Lds32Pos;
+    static constexpr bool STAGE_Q2 = true;
+    static constexpr int WAVES = 5;
+    struct Tgt {
+        float x, y, z;
+    };
+    struct Part {
+        unsigned long long h, n;
+    };
+    using Acc = Part;
+    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+                                double oz, Tgt& t)
+    {
+        t.x = (float)(COL(c, C_X, p) - ox);
+        t.y = (float)(COL(c, C_Y, p) - oy);
+        t.z = (float)(COL(c, C_Z, p) - oz);
+    }
+    static __device__ bool active(const Tgt&) { return true; }
+    template <bool P>
+    static __device__ Raw fetch(const PairCtx& c, int q, const Seg& g)
+    {
+        Raw r{};
+        fetch_pos<P>(c, q, g, r);
+        return r;
+    }
+    template <bool FLAT>
+    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+                                    double oz, Lds& s)
+    {
+        s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
+    }
+    static __device__ void init(Acc& a) { a.h = a.n = 0; }
+    static __device__ void zero(Part& a) { a.h = a.n = 0; }
+    static constexpr int NVAR = 1;
+    static __device__ int variant(const SysDesc&) { return 0; }
+    template <int V>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+                                    int, int, int q, float, float, float, float, Part& a)
+    {
+        a.h += mix64((unsigned long long)c.refidx[q] + 1ull);
+        a.n += 1;
+    }
+    static __device__ void fold(Acc& a, const Part& q) { a.h += q.h; a.n += q.n; }
+    static __device__ void finish(const PairCtx& c, const SysDesc&, int p, const Tgt&,
+                                  const Acc& a, int T, bool write, int)
+    {
+        digest_write(c, p, a.h, a.n, T, write);
+    }
 };
 
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
@@ -1384,7 +1666,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     const double ox = ld_const(&COL(c, C_X, first)), oy = ld_const(&COL(c, C_Y, first)),
                  oz = ld_const(&COL(c, C_Z, first));
     typename Op::Tgt tg;
-    Op::load(c, pp, ox, oy, oz, tg);
+    Op::load(c, d, pp, ox, oy, oz, tg);
     const bool act = valid && Op::active(tg);
     if (!__any(act)) return;
     const double s_thr = d.g.s_thr;
@@ -1433,8 +1715,8 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 const int q = c.kept[lbase + k];
                 Seg sh{};
                 if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
-                Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & 127, ox, oy,
-                                       oz, lds);
+                Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & 127, d, ox,
+                                       oy, oz, lds);
                 L.q[k & 127] = q;
             }
             fill = min(nk, head + 64);
@@ -1452,7 +1734,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                                (unsigned)(rq - first) < (unsigned)nvalid);
             const int pos = compact(L, fill, nself, keep, rq, rrg, first);
             if (pos >= 0) {
-                Op::template put<FLAT>(raw, pos & 127, ox, oy, oz, lds);
+                Op::template put<FLAT>(raw, pos & 127, d, ox, oy, oz, lds);
                 if (store_kept) {
                     c.kept[lbase + pos] = rq;
                     if constexpr (PERIODIC) c.kept_rg[lbase + pos] = (unsigned char)rrg;
@@ -1466,7 +1748,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 // the last chunk: pad its free slots
                 const int k = head + lane;
                 if (k >= fill) {
-                    Op::template put<FLAT>(pad_raw(), k & 127, ox, oy, oz, lds);
+                    Op::template put<FLAT>(pad_raw(), k & 127, d, ox, oy, oz, lds);
                     L.q[k & 127] = first;
                     if constexpr (PERIODIC) L.rg[k & 127] = 0;
                 }
@@ -1646,6 +1928,7 @@ static void launch_tile(int op, int precision, const PairCtx& c, int grid, hipSt
         case P_CONTINUITY: k_pair64<ROWS, FLAT, PER, T, Continuity64><<<grid, 64, 0, st>>>(c); break;
         case P_COUNT: k_pair64<ROWS, FLAT, PER, T, Pairs64<false>><<<grid, 64, 0, st>>>(c); break;
         case P_EMIT: k_pair64<ROWS, FLAT, PER, T, Pairs64<true>><<<grid, 64, 0, st>>>(c); break;
+        case P_DIGEST: k_pair64<ROWS, FLAT, PER, T, Digest64><<<grid, 64, 0, st>>>(c); break;
         }
     } else {
         switch (op) {
@@ -1654,9 +1937,9 @@ static void launch_tile(int op, int precision, const PairCtx& c, int grid, hipSt
         case P_FORCE: k_pair32<ROWS, FLAT, PER, T, Force32><<<grid, 64, 0, st>>>(c, 0); break;
         case P_COUNT: k_pair32<ROWS, FLAT, PER, T, Pairs32<false>><<<grid, 64, 0, st>>>(c, 0); break;
         case P_EMIT: k_pair32<ROWS, FLAT, PER, T, Pairs32<true>><<<grid, 64, 0, st>>>(c, 0); break;
-        // operators without a mixed form run in FP64
-        case P_FORCE_RHO0: k_pair64<ROWS, FLAT, PER, T, ForceRho064><<<grid, 64, 0, st>>>(c); break;
-        case P_CONTINUITY: k_pair64<ROWS, FLAT, PER, T, Continuity64><<<grid, 64, 0, st>>>(c); break;
+        case P_DIGEST: k_pair32<ROWS, FLAT, PER, T, Digest32><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_FORCE_RHO0: k_pair32<ROWS, FLAT, PER, T, ForceRho032><<<grid, 64, 0, st>>>(c, 0); break;
+        case P_CONTINUITY: k_pair32<ROWS, FLAT, PER, T, Continuity32><<<grid, 64, 0, st>>>(c, 0); break;
         }
     }
 }

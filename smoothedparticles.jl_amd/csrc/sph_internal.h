@@ -141,8 +141,12 @@ enum PairOp {
     P_CONTINUITY = 4,    // balance_of_mass!
     P_COUNT = 5,         // accepted pair count
     P_EMIT = 6,          // pair emission
+    P_DIGEST = 7,        // per-target neighbour count and hash
     NPAIROP
 };
+
+// Unary stages of the collapse_dry time loop (k_cd_unary).
+enum CdStage { CD_KICK_MOVE = 0, CD_MOVE = 1, CD_KICK = 2 };
 
 // One contiguous candidate range of a stencil row, with the periodic image
 // shift added to the candidates' coordinates.

@@ -48,7 +48,9 @@ __global__ __launch_bounds__(256) void k_unary(int op, const int* meta, double* 
         rho = rho + UCOL(cols, cap, C_DRHO, i) * ph.dt;
         UCOL(cols, cap, C_RHO, i) = rho;
         UCOL(cols, cap, C_DRHO, i) = 0.0;
-        UCOL(cols, cap, C_P, i) = (ph.c * ph.c) * (rho - ph.rho0);
+        const double P = (ph.c * ph.c) * (rho - ph.rho0);
+        UCOL(cols, cap, C_P, i) = P;
+        if (prho2) prho2[i] = P / (rho * rho);
         break;
     }
     case SPH_OP_ACCELERATE: {  // examples/collapse_dry.jl:155-159
@@ -60,10 +62,26 @@ __global__ __launch_bounds__(256) void k_unary(int op, const int* meta, double* 
         break;
     }
     case SPH_OP_MOVE: {  // examples/collapse_dry.jl:148-153
+        // p.Dv = VEC0 for every particle, then the fluid drift
+        UCOL(cols, cap, C_AX, i) = 0.0;
+        UCOL(cols, cap, C_AY, i) = 0.0;
+        UCOL(cols, cap, C_AZ, i) = 0.0;
         if (UCOL(cols, cap, C_TYPE, i) != 0.0) break;
         const double k = ph.coeff * ph.dt;
         for (int d = 0; d < 3; ++d)
             UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * UCOL(cols, cap, C_VX + d, i);
+        break;
+    }
+    case SPH_OP_DRIFT: {  // tests/test_collision_2d.jl:88-90: p.x += dt*p.v
+        const double k = ph.coeff * ph.dt;
+        for (int d = 0; d < 3; ++d)
+            UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * UCOL(cols, cap, C_VX + d, i);
+        break;
+    }
+    case SPH_OP_KICK: {  // tests/test_collision_2d.jl:92-94: p.v += 0.5*dt*p.a
+        const double k = ph.coeff * ph.dt;
+        for (int d = 0; d < 3; ++d)
+            UCOL(cols, cap, C_VX + d, i) = UCOL(cols, cap, C_VX + d, i) + k * UCOL(cols, cap, C_AX + d, i);
         break;
     }
     default:
@@ -100,6 +118,38 @@ __global__ __launch_bounds__(256) void k_kick_drift(const int* meta, double* col
         const double v = UCOL(cols, cap, C_VX + d, i) + k * (UCOL(cols, cap, C_AX + d, i) + ph.g[d]);
         UCOL(cols, cap, C_VX + d, i) = v;
         UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * v;
+    }
+}
+
+// The unary stages of the time loop of examples/collapse_dry.jl:202-211,
+// with the half step k = 0.5*dt of its move! and accelerate! (:148-159):
+//   CD_KICK_MOVE  accelerate! then move!: fluid v += k (a + g); a = 0 for
+//                 every particle; fluid x += k v
+//   CD_MOVE       move!: a = 0; fluid x += k v
+//   CD_KICK       accelerate!: fluid v += k (a + g)
+// The continuity find_pressure! between them is SPH_OP_FIND_PRESSURE_CONTINUITY.
+__global__ __launch_bounds__(256) void k_cd_unary(int stage, const int* meta, double* cols,
+                                                  long long cap, const SysDesc* __restrict__ sysd,
+                                                  const int* __restrict__ sysid)
+{
+    const int n = meta[META_N];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const Phys& ph = sysd[sysid[i]].ph;
+    const double k = 0.5 * ph.dt;
+    const bool fluid = UCOL(cols, cap, C_TYPE, i) == 0.0;
+    double v[3];
+    for (int d = 0; d < 3; ++d) {
+        v[d] = UCOL(cols, cap, C_VX + d, i);
+        if (fluid && stage != CD_MOVE) {
+            v[d] = v[d] + k * (UCOL(cols, cap, C_AX + d, i) + ph.g[d]);
+            UCOL(cols, cap, C_VX + d, i) = v[d];
+        }
+    }
+    if (stage == CD_KICK) return;
+    for (int d = 0; d < 3; ++d) {
+        UCOL(cols, cap, C_AX + d, i) = 0.0;
+        if (fluid) UCOL(cols, cap, C_X + d, i) = UCOL(cols, cap, C_X + d, i) + k * v[d];
     }
 }
 
@@ -385,6 +435,16 @@ __global__ __launch_bounds__(256) void k_field_out(int off, int n, const int* __
     for (int d = 0; d < ncomp; ++d) out[ncomp * r + d] = UCOL(cols, cap, col0 + d, k);
 }
 
+// find_key of each particle at the last build, 1-based within its system.
+__global__ __launch_bounds__(256) void k_key_out(int off, int n, const int* __restrict__ ref,
+                                                 const int* __restrict__ key, int cell_base,
+                                                 double* out)
+{
+    const int k = off + blockIdx.x * 256 + threadIdx.x;
+    if (k >= off + n) return;
+    out[ref[k]] = (double)(key[k] - cell_base + 1);
+}
+
 __global__ __launch_bounds__(256) void k_field_in(int off, int n, const int* __restrict__ ref,
                                                   double* cols, long long cap, int col0, int ncomp,
                                                   const double* in)
@@ -459,6 +519,13 @@ void enqueue_prho2(sph_system* s)
     k_prho2<<<nblk(s->n), 256, 0, s->stream>>>(s->meta, s->cols[s->cur], s->cap, s->prho2);
 }
 
+void enqueue_cd_unary(sph_system* s, int stage)
+{
+    if (s->n <= 0) return;
+    k_cd_unary<<<nblk(s->n), 256, 0, s->stream>>>(stage, s->meta, s->cols[s->cur], s->cap, s->sysd,
+                                                 sysid_col(s));
+}
+
 void enqueue_kick_drift(sph_system* s)
 {
     if (s->n <= 0) return;
@@ -493,6 +560,13 @@ void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int 
     if (n <= 0) return;
     k_field_out<<<nblk(n), 256, 0, s->stream>>>((int)off, (int)n, s->icols[s->cur] + I_REF * s->cap,
                                                s->cols[s->cur], s->cap, col0, ncomp, out_dev);
+}
+
+void enqueue_key_out(sph_system* s, long long off, long long n, int cell_base, double* out_dev)
+{
+    if (n <= 0) return;
+    k_key_out<<<nblk(n), 256, 0, s->stream>>>((int)off, (int)n, s->icols[s->cur] + I_REF * s->cap,
+                                             s->icols[s->cur] + I_KEY * s->cap, cell_base, out_dev);
 }
 
 void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,

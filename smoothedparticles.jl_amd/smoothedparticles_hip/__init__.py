@@ -16,8 +16,12 @@ Reference call surface and its counterpart here:
 independent systems, periodic ones included, and steps them together.
 
 Operators cannot be arbitrary closures across the C ABI; the built-in ones
-are registered by the reference name they replace (``OPERATORS``).
-Everything runs through the HIP library; there is no CPU fallback.
+are registered by the reference name they replace, per reference program
+(``PROGRAMS``; ``OPERATORS`` holds the names that are unambiguous).  A Python
+callable passed to apply/apply_binary/apply_unary runs through the host
+fallback (``host_apply``): the device finds the pairs, the closure runs on
+the host over downloaded fields.  Every registered operator runs in the HIP
+library; if the library is missing, every call raises.
 """
 import ctypes
 
@@ -34,7 +38,9 @@ __all__ = [
     "internal_force_rho0", "balance_of_mass", "reset_rho", "reset_a", "find_pressure",
     "find_pressure_rho0", "find_pressure_continuity", "accelerate", "move",
     "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
-    "probe_sums", "assemble_vector", "assemble_matrix",
+    "probe_sums", "assemble_vector", "assemble_matrix", "drift", "kick", "PROGRAMS",
+    "program_operators", "host_apply", "Particle", "neighbour_digest", "cell_keys",
+    "visit_order",
 ]
 
 FLUID = 0.0
@@ -61,12 +67,16 @@ def Rectangle(x1_min, x2_min, x1_max, x2_max):
 
 
 class Operator:
-    """A registered operator: its library id and whether it is binary."""
+    """A registered operator: its library id, whether it is binary, and the
+    parameters the reference function fixes in its body (for example the
+    0.5*dt half step of collapse_dry.jl's move!), which override the call's
+    keyword parameters."""
 
-    def __init__(self, name, op_id, binary):
+    def __init__(self, name, op_id, binary, fixed=None):
         self.name = name
         self.id = op_id
         self.binary = binary
+        self.fixed = dict(fixed or {})
 
     def __repr__(self):
         return f"<operator {self.name}>"
@@ -85,22 +95,64 @@ find_pressure_continuity = Operator(
     "find_pressure! (continuity)", N.SPH_OP_FIND_PRESSURE_CONTINUITY, False)
 accelerate = Operator("accelerate!", N.SPH_OP_ACCELERATE, False)
 move = Operator("move!", N.SPH_OP_MOVE, False)
+drift = Operator("move! (all particles, a kept)", N.SPH_OP_DRIFT, False)
+kick = Operator("accelerate! (all particles, no gravity)", N.SPH_OP_KICK, False)
 
-# Reference operator name -> registered operator.
-OPERATORS = {
-    "find_rho!": find_rho,                       # tests/test_collision_2d.jl:63-65
-    "find_rho0!": find_rho0,                     # tests/test_collision_2d.jl:67-69
-    "internal_force!": internal_force,           # examples/collapse_dry.jl:135-141
-    "internal_force_rho0!": internal_force_rho0,  # tests/test_collision_2d.jl:75-78
-    "balance_of_mass!": balance_of_mass,         # examples/collapse_dry.jl:112-115
-    "reset_rho!": reset_rho,                     # tests/test_collision_2d.jl:84-86
-    "reset_a!": reset_a,                         # tests/test_collision_2d.jl:80-82
-    "find_pressure!": find_pressure,             # examples/collapse_dry.jl:126
-    "find_pressure_rho0!": find_pressure_rho0,   # tests/test_collision_2d.jl:71-73
-    "find_pressure_continuity!": find_pressure_continuity,  # examples/collapse_dry.jl:123-127
-    "accelerate!": accelerate,                   # examples/collapse_dry.jl:155-159
-    "move!": move,                               # examples/collapse_dry.jl:148-153
+# The operators of each reference program, by the name the program gives
+# them.  Several names (find_pressure!, internal_force!, move!,
+# accelerate!) have different bodies in different programs, so a bare name
+# is resolved only where it is unambiguous; otherwise the program is named
+# ("collapse_dry:move!") or the table below is used directly.
+PROGRAMS = {
+    "collapse_dry": {  # examples/collapse_dry.jl
+        "balance_of_mass!": balance_of_mass,                         # :112-115
+        "find_pressure!": find_pressure_continuity,                  # :123-127
+        "internal_force!": internal_force,                           # :135-141
+        "move!": Operator("move!", N.SPH_OP_MOVE, False, {"coeff": 0.5}),           # :148-153
+        "accelerate!": Operator("accelerate!", N.SPH_OP_ACCELERATE, False, {"coeff": 0.5}),  # :155-159
+    },
+    "test_collision_2d": {  # tests/test_collision_2d.jl
+        "find_rho!": find_rho,                                       # :63-65
+        "find_rho0!": find_rho0,                                     # :67-69
+        "find_pressure!": find_pressure_rho0,                        # :71-73
+        "internal_force!": internal_force_rho0,                      # :75-78
+        "reset_a!": reset_a,                                         # :80-82
+        "reset_rho!": reset_rho,                                     # :84-86
+        "move!": Operator("move!", N.SPH_OP_DRIFT, False, {"coeff": 1.0}),          # :88-90
+        "accelerate!": Operator("accelerate!", N.SPH_OP_KICK, False, {"coeff": 0.5}),  # :92-94
+    },
 }
+
+
+def _bare_names():
+    """Names that mean the same operator in every program that defines
+    them, plus the library's own unambiguous names."""
+    by_name = {}
+    for ops in PROGRAMS.values():
+        for name, op in ops.items():
+            by_name.setdefault(name, set()).add((op.id, tuple(sorted(op.fixed.items()))))
+    table = {name: next(op for ops in PROGRAMS.values() for n, op in ops.items() if n == name)
+             for name, ids in by_name.items() if len(ids) == 1}
+    table.update({
+        "internal_force_rho0!": internal_force_rho0,
+        "find_pressure_rho0!": find_pressure_rho0,
+        "find_pressure_continuity!": find_pressure_continuity,
+        "find_pressure_constant!": find_pressure,   # P = c^2 (rho - rho0), collapse_dry.jl:126
+    })
+    return table
+
+
+# Unambiguous name -> registered operator.
+OPERATORS = _bare_names()
+AMBIGUOUS = sorted({n for ops in PROGRAMS.values() for n in ops} - set(OPERATORS))
+
+
+def program_operators(program):
+    """The operators of one reference program, by its own names."""
+    if program not in PROGRAMS:
+        raise KeyError(f"unknown program {program!r}; known: {sorted(PROGRAMS)}")
+    return dict(PROGRAMS[program])
+
 
 _PARAM_NAMES = ("m", "rho0", "c", "alpha", "hs", "mu", "nu", "g", "dt", "coeff")
 
@@ -290,38 +342,187 @@ def create_cell_list(sys):
 
 
 def _resolve(op):
+    """A registered operator for an Operator, a name or "program:name";
+    None for a Python callable (it runs through the host fallback)."""
     if isinstance(op, Operator):
         return op
-    if isinstance(op, str) and op in OPERATORS:
-        return OPERATORS[op]
-    raise NotImplementedError(
-        f"{op!r} is not a registered operator; only the built-in operators "
-        f"{sorted(OPERATORS)} run on the device")
+    if isinstance(op, str):
+        if ":" in op:
+            prog, name = op.split(":", 1)
+            ops = program_operators(prog)
+            if name not in ops:
+                raise KeyError(f"{prog} defines no operator {name!r}")
+            return ops[name]
+        if op in OPERATORS:
+            return OPERATORS[op]
+        if op in AMBIGUOUS:
+            progs = sorted(p for p, ops in PROGRAMS.items() if op in ops)
+            raise KeyError(
+                f"{op!r} names different operators in {progs}; write "
+                f"'{progs[0]}:{op}' or use program_operators()")
+        raise KeyError(f"{op!r} is not a registered operator")
+    if callable(op):
+        return None
+    raise TypeError(f"{op!r} is neither an operator nor a callable")
+
+
+def _params(op, params):
+    merged = dict(params)
+    merged.update(op.fixed)
+    return make_params(**merged)
 
 
 def apply_binary(sys, op, **params):
-    """apply_binary!(sys, op) -- src/core.jl:125-129."""
-    op = _resolve(op)
-    if not op.binary:
-        raise TypeError(f"{op.name} is unary")
-    p = make_params(**params)
-    check(lib().sph_apply_binary(sys._h, op.id, ctypes.byref(p), 0))
+    """apply_binary!(sys, op) -- src/core.jl:125-129.  A Python callable
+    op(p, q, r) runs through the host fallback (host_apply)."""
+    reg = _resolve(op)
+    if reg is None:
+        return host_apply(sys, op, binary=True)
+    if not reg.binary:
+        raise TypeError(f"{reg.name} is unary")
+    p = _params(reg, params)
+    check(lib().sph_apply_binary(sys._h, reg.id, ctypes.byref(p), 0))
 
 
 def apply_unary(sys, op, **params):
-    """apply_unary!(sys, op) -- src/core.jl:138-142."""
-    op = _resolve(op)
-    if op.binary:
-        raise TypeError(f"{op.name} is binary")
-    p = make_params(**params)
-    check(lib().sph_apply_unary(sys._h, op.id, ctypes.byref(p)))
+    """apply_unary!(sys, op) -- src/core.jl:138-142.  A Python callable
+    op(p) runs through the host fallback (host_apply)."""
+    reg = _resolve(op)
+    if reg is None:
+        return host_apply(sys, op, binary=False)
+    if reg.binary:
+        raise TypeError(f"{reg.name} is binary")
+    p = _params(reg, params)
+    check(lib().sph_apply_unary(sys._h, reg.id, ctypes.byref(p)))
 
 
 def apply(sys, op, self=False, **params):
-    """apply!(sys, op; self=false, parameters...) -- src/core.jl:151-161."""
-    op = _resolve(op)
-    p = make_params(**params)
-    check(lib().sph_apply(sys._h, op.id, ctypes.byref(p), 1 if self else 0))
+    """apply!(sys, op; self=false, parameters...) -- src/core.jl:151-161.
+    A Python callable is dispatched as the reference does
+    (hasmethod(op, (T, T, Float64)), :152): three positional parameters make
+    it binary, one unary; it runs through the host fallback."""
+    reg = _resolve(op)
+    if reg is None:
+        return host_apply(sys, op, self=self, **params)
+    p = _params(reg, params)
+    check(lib().sph_apply(sys._h, reg.id, ctypes.byref(p), 1 if self else 0))
+
+
+class Particle:
+    """One particle of the host fallback: attribute access to its fields in
+    the downloaded arrays (x, v, a -- also Dv --, rho, P, Drho, rho0,
+    type).  Vector fields read as copies and are written whole, as the
+    reference's immutable RealVector fields are (p.x += ... rebinds)."""
+
+    __slots__ = ("_f", "_i")
+    _ALIAS = {"Dv": "a"}
+
+    def __init__(self, fields, i):
+        object.__setattr__(self, "_f", fields)
+        object.__setattr__(self, "_i", i)
+
+    def __getattr__(self, name):
+        f = self._f[self._ALIAS.get(name, name)]
+        v = f[self._i]
+        return v.copy() if f.ndim == 2 else float(v)
+
+    def __setattr__(self, name, value):
+        self._f[self._ALIAS.get(name, name)][self._i] = value
+
+    def __eq__(self, other):  # identity, as == on a mutable struct
+        return isinstance(other, Particle) and other._f is self._f and other._i == self._i
+
+    def __hash__(self):
+        return hash(self._i)
+
+
+def _arity(func):
+    import inspect
+
+    try:
+        sig = inspect.signature(func)
+    except (TypeError, ValueError):
+        return None
+    pos = [p for p in sig.parameters.values()
+           if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD) and p.default is p.empty]
+    return len(pos)
+
+
+def visit_order(sys):
+    """The accepted pairs of the current cell list in the order
+    _apply_binary! visits them (src/core.jl:94-112): by target p, then by
+    the key_diff entry of q's cell (di outermost, :70-82 of
+    src/structs.jl), then by descending q (the order add_index! keeps,
+    src/core.jl:26-41).  Pairs and cell keys come from the device; the
+    order is a sort on the host."""
+    pairs = neighbour_pairs(sys)
+    if len(pairs) == 0:
+        return pairs
+    n = len(sys)
+    keys = np.empty(n, dtype=np.float64)
+    check(lib().sph_get_field(sys._h, N.SPH_FIELD_KEY, _as_ptr(keys), n))
+    keys = keys.astype(np.int64)
+    diff = np.asarray(sys.key_geometry()[3], dtype=np.int64)
+    delta = keys[pairs[:, 1]] - keys[pairs[:, 0]]
+    # position of each pair's key difference in key_diff; a difference that
+    # key_diff holds more than once (aliased grids) is visited once per entry,
+    # and the device then lists the pair once per visit
+    order_of = {}
+    for i, d in enumerate(diff):
+        order_of.setdefault(int(d), []).append(i)
+    rank = np.empty(len(pairs), dtype=np.int64)
+    idx = np.lexsort((-pairs[:, 1], delta, pairs[:, 0]))
+    pairs, delta = pairs[idx], delta[idx]
+    start = 0
+    while start < len(pairs):
+        end = start + 1
+        while (end < len(pairs) and pairs[end, 0] == pairs[start, 0]
+               and pairs[end, 1] == pairs[start, 1]):
+            end += 1
+        slots = order_of[int(delta[start])]
+        rank[start:end] = slots[: end - start]
+        start = end
+    idx = np.lexsort((-pairs[:, 1], rank, pairs[:, 0]))
+    return pairs[idx]
+
+
+def host_apply(sys, func, self=False, binary=None, **params):
+    """The host fallback for an operator that is not registered (a Python
+    callable): download every field, run func over the particles (unary,
+    func(p)) or over the accepted pairs of the current cell list in the
+    reference's visiting order (binary, func(p, q, r)), then upload every
+    field.  With self=True a binary func also gets func(p, p, 0.0) for each
+    p after the pair pass (src/core.jl:155-157).  Keyword parameters are
+    passed on to func, as apply! forwards them (:154).  Only the fields the
+    closure changed are uploaded.  Correct but slow: the pairs are found on
+    the device, the closure runs in Python."""
+    if binary is None:
+        k = _arity(func)
+        if k not in (1, 3):
+            raise TypeError("an operator takes (p) or (p, q, r)")
+        binary = k == 3
+    n = len(sys)
+    fields = {k: sys.get(k) for k in N.FIELDS}
+    before = {k: a.copy() for k, a in fields.items()}
+    parts = [Particle(fields, i) for i in range(n)]
+    call = (lambda *a: func(*a, **params)) if params else func
+    if binary:
+        x = fields["x"]
+        pairs = visit_order(sys)
+        for p, q in pairs:
+            d = x[p] - x[q]
+            r = float(np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
+            call(parts[p], parts[q], r)
+        if self:
+            for i in range(n):
+                call(parts[i], parts[i], 0.0)
+    else:
+        for i in range(n):
+            call(parts[i])
+    # upload what the closure changed (an unchanged x keeps the cell list)
+    for k, a in fields.items():
+        if not np.array_equal(a, before[k], equal_nan=True):
+            sys.set(k, a)
 
 
 def _reduce(sys, what, nout, **params):
@@ -343,6 +544,26 @@ def energy(sys, kind="wcsph", **params):
 def momentum(sys, **params):
     """Sum of m v over all particles."""
     return np.array(_reduce(sys, N.SPH_REDUCE_MOMENTUM, 3, **params))
+
+
+def neighbour_digest(sys):
+    """(counts, hashes) of every particle's neighbour set under the current
+    cell list: the number of accepted q and the sum modulo 2^64 of
+    mix64(q + 1) over them (include/sph.h, sph_neighbour_digest)."""
+    n = len(sys)
+    c = np.empty(max(n, 1), dtype=np.int64)
+    hsh = np.empty(max(n, 1), dtype=np.uint64)
+    check(lib().sph_neighbour_digest(sys._h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n))
+    return c[:n], hsh[:n]
+
+
+def cell_keys(sys):
+    """find_key of each particle at the last cell-list build (1-based)."""
+    n = len(sys)
+    keys = np.empty(n, dtype=np.float64)
+    check(lib().sph_get_field(sys._h, N.SPH_FIELD_KEY, _as_ptr(keys), n))
+    return keys.astype(np.int64)
 
 
 def neighbour_pairs(sys):
@@ -414,16 +635,31 @@ def assemble_matrix(sys, func):
     return scipy.sparse.csr_matrix((V, (I, J)), shape=(n, n))
 
 
+_SCHEMES = {"wcsph_se": N.SPH_SCHEME_WCSPH_SE, "wcsph_se_rho0": N.SPH_SCHEME_WCSPH_SE_RHO0,
+            "collapse_dry": N.SPH_SCHEME_COLLAPSE_DRY}
+
+
 def step(sys, nsteps, scheme="wcsph_se", **params):
-    """nsteps fused device-resident steps (density sum, EOS, internal force,
-    symplectic Euler kick and drift).  scheme "wcsph_se" uses the constant
-    rho0 in the EOS, "wcsph_se_rho0" the per-particle rho0 field.  Returns
-    the accepted ordered pairs summed over the density passes."""
-    sid = {"wcsph_se": N.SPH_SCHEME_WCSPH_SE, "wcsph_se_rho0": N.SPH_SCHEME_WCSPH_SE_RHO0}[scheme]
+    """nsteps fused device-resident steps.  scheme "wcsph_se" (density sum,
+    EOS with the constant rho0, internal force, symplectic Euler kick and
+    drift), "wcsph_se_rho0" (the same with the per-particle rho0 field), or
+    "collapse_dry" (the time loop of examples/collapse_dry.jl:202-211:
+    half-step kick and drift, rebuild, balance_of_mass!, continuity EOS,
+    half drift, rebuild, internal force, half kick; call collapse_dry_init
+    first).  Returns the accepted ordered pairs summed over the first pair
+    pass of each step (density sum, or balance_of_mass!)."""
+    sid = _SCHEMES[scheme]
     p = make_params(**params)
     pairs = ctypes.c_int64()
     check(lib().sph_step(sys._h, sid, ctypes.byref(p), int(nsteps), ctypes.byref(pairs)))
     return pairs.value
+
+
+def collapse_dry_init(sys, **params):
+    """create_cell_list!(sys); apply!(sys, internal_force!): the set-up of
+    the time loop of examples/collapse_dry.jl:198-200."""
+    create_cell_list(sys)
+    apply(sys, internal_force, **params)
 
 
 def init_reference_density(sys, **params):
@@ -434,7 +670,6 @@ def init_reference_density(sys, **params):
     apply(sys, find_rho0, self=True, **params)
 
 
-_SCHEMES = {"wcsph_se": N.SPH_SCHEME_WCSPH_SE, "wcsph_se_rho0": N.SPH_SCHEME_WCSPH_SE_RHO0}
 
 
 class Batch:
@@ -548,6 +783,16 @@ class Batch:
         check(lib().sph_batch_probe_sum(self._h, int(sid), flags, xs.ctypes.data_as(_dptr),
                                         xs.shape[0], out.ctypes.data_as(_dptr)))
         return out
+
+    def neighbour_digest(self, sid):
+        """neighbour_digest of one system under the current cell lists."""
+        n = self.count(sid)
+        c = np.empty(max(n, 1), dtype=np.int64)
+        hsh = np.empty(max(n, 1), dtype=np.uint64)
+        check(lib().sph_batch_neighbour_digest(
+            self._h, int(sid), c.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            hsh.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n))
+        return c[:n], hsh[:n]
 
     def neighbour_pairs(self, sid):
         """Accepted ordered pairs of one system under the current cell lists

@@ -24,6 +24,7 @@ SPH_MIXED = 1
 
 FIELDS = {"x": 0, "v": 1, "a": 2, "rho": 3, "P": 4, "Drho": 5, "rho0": 6, "type": 7}
 VECTOR_FIELDS = {"x", "v", "a"}
+SPH_FIELD_KEY = 8  # read only: find_key at the last build (1-based)
 
 # operator ids
 SPH_OP_DENSITY_SUM = 1
@@ -38,6 +39,8 @@ SPH_OP_FIND_PRESSURE_RHO0 = 19
 SPH_OP_FIND_PRESSURE_CONTINUITY = 20
 SPH_OP_ACCELERATE = 21
 SPH_OP_MOVE = 22
+SPH_OP_DRIFT = 23
+SPH_OP_KICK = 24
 
 SPH_REDUCE_ENERGY = 0
 SPH_REDUCE_ENERGY_COLLISION = 1
@@ -46,6 +49,7 @@ SPH_REDUCE_ENERGY_RHO0 = 3
 
 SPH_SCHEME_WCSPH_SE = 0
 SPH_SCHEME_WCSPH_SE_RHO0 = 1
+SPH_SCHEME_COLLAPSE_DRY = 2
 
 SPH_OPTION_PAIR_MASKS = 0
 SPH_OPTION_KEPT_LISTS = 1
@@ -104,6 +108,7 @@ SIGNATURES = {
     "sph_apply": (_i, [_p, _i, ctypes.POINTER(sph_params), _i]),
     "sph_reduce": (_i, [_p, _i, ctypes.POINTER(sph_params), _pd]),
     "sph_neighbour_pairs": (_i, [_p, _pi64, _i64, _pi64]),
+    "sph_neighbour_digest": (_i, [_p, _pi64, ctypes.POINTER(ctypes.c_uint64), _i64]),
     "sph_probe_sum": (_i, [_p, ctypes.POINTER(sph_params), _i, _pd, _i64, _pd]),
     "sph_batch_probe_sum": (_i, [_p, _i, _i, _pd, _i64, _pd]),
     "sph_step": (_i, [_p, _i, ctypes.POINTER(sph_params), _i, _pi64]),
@@ -129,6 +134,7 @@ SIGNATURES = {
     "sph_batch_reduce": (_i, [_p, _i, _i, _pd]),
     "sph_batch_build_cells": (_i, [_p]),
     "sph_batch_neighbour_pairs": (_i, [_p, _i, _pi64, _i64, _pi64]),
+    "sph_batch_neighbour_digest": (_i, [_p, _i, _pi64, ctypes.POINTER(ctypes.c_uint64), _i64]),
     "sph_batch_init_rho0": (_i, [_p]),
     "sph_batch_step": (_i, [_p, _i, _i, _pi64]),
     "sph_batch_set_tile": (_i, [_p, _i]),

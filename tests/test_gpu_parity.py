@@ -201,23 +201,28 @@ def test_internal_force(precision, tol, dim, mu):
     assert np.all(got[typ != 0.0] == 0.0)  # walls receive no force
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_pressure_force_rho0_and_continuity(precision):
-    n = 2500
-    x, lo, hi, rng = random_cloud(n, 2, 41)
-    v, rho, P, typ = fields(rng, n, 2)
-    g, o = make_both(x, 0.06, lo, hi, precision, v, rho, P, typ)
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 1e-5)])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pressure_force_rho0_and_continuity(precision, tol, dim):
+    """internal_force! with rho0^2 (tests/test_collision_2d.jl:75-78) and
+    balance_of_mass! (examples/collapse_dry.jl:112-115); the mixed forms are
+    FP32 pair arithmetic with compensated sums: within 1e-5 of max|a| and
+    max|Drho|, FP64 within 1e-11."""
+    n = 2500 if dim == 2 else 4000
+    x, lo, hi, rng = random_cloud(n, dim, 41)
+    v, rho, P, typ = fields(rng, n, dim)
+    g, o = make_both(x, 0.06 if dim == 2 else 0.12, lo, hi, precision, v, rho, P, typ)
     sp.create_cell_list(g)
     o.create_cell_list()
     kw = dict(PHYS, nu=1e-3)
     sp.apply(g, sp.internal_force_rho0, **kw)
     o.pressure_force_rho0(**kw)
     ref = o.field("a")
-    assert np.max(np.abs(g.get("a") - ref)) <= 1e-11 * np.abs(ref).max()
+    assert np.max(np.abs(g.get("a") - ref)) <= tol * np.abs(ref).max()
     sp.apply(g, sp.balance_of_mass, **kw)
     o.balance_of_mass(**kw)
     ref = o.field("Drho")
-    assert np.max(np.abs(g.get("Drho") - ref)) <= 1e-11 * np.abs(ref).max()
+    assert np.max(np.abs(g.get("Drho") - ref)) <= tol * np.abs(ref).max()
 
 
 def test_unary_operators_bit_exact():
@@ -233,7 +238,15 @@ def test_unary_operators_bit_exact():
     o.field("Drho")[:] = d
     kw = dict(PHYS, coeff=0.5, g=(0.1, -9.8, 0.3))
     sp.apply(g, sp.accelerate, **kw); o.accelerate(**kw)
+    # test_collision_2d.jl's move! and accelerate!: every particle, a kept
+    sp.apply(g, sp.drift, **kw); o.drift(**kw)
+    sp.apply(g, sp.kick, **kw); o.kick(**kw)
+    for f in ("x", "v", "a"):
+        assert np.array_equal(g.get(f), o.field(f)), f
+    assert np.array_equal(g.get("a"), a)
+    # collapse_dry.jl's move!: p.Dv = VEC0 for every particle, fluid drift
     sp.apply(g, sp.move, **kw); o.move(**kw)
+    assert np.all(g.get("a") == 0.0) and np.all(o.field("a") == 0.0)
     sp.apply(g, sp.find_pressure, **kw); o.eos(**kw)
     for f in ("x", "v", "P"):
         assert np.array_equal(g.get(f), o.field(f)), f

@@ -55,9 +55,12 @@ def test_fused_step_equals_operator_sequence():
         sp.apply(g2, sp.reset_a)
         sp.apply(g2, sp.internal_force, **p)
         sp.apply(g2, sp.accelerate, **p)
-        sp.apply(g2, sp.move, **p)
-    for f in ("x", "v", "rho", "P", "a"):
+        a_last = g2.get("a")
+        sp.apply(g2, sp.move, **p)   # resets a (collapse_dry.jl:149), the fused step keeps it
+        assert np.all(g2.get("a") == 0.0)
+    for f in ("x", "v", "rho", "P"):
         assert np.array_equal(g1.get(f), g2.get(f)), f
+    assert np.array_equal(g1.get("a"), a_last)
 
 
 def test_dambreak_2d_fp64_100_steps():
@@ -117,16 +120,18 @@ def test_collision_2d_energy_on_device():
     steps = int(round(1.0 / st.dt))
     every = int(round(0.1 / st.dt))
     N, E = [], []
+    ops = sp.program_operators("test_collision_2d")
     for k in range(steps + 1):
-        sp.apply(g, sp.accelerate, coeff=0.5, **kw)
-        sp.apply(g, sp.move, coeff=1.0, **kw)
+        # verlet_step! (tests/test_collision_2d.jl:106-116), by the program's own names
+        sp.apply(g, ops["accelerate!"], **kw)
+        sp.apply(g, ops["move!"], **kw)
         sp.create_cell_list(g)
-        sp.apply(g, sp.reset_rho)
-        sp.apply(g, sp.find_rho, self=True, **kw)
-        sp.apply(g, sp.find_pressure_rho0, **kw)
-        sp.apply(g, sp.reset_a)
-        sp.apply(g, sp.internal_force_rho0, **kw)
-        sp.apply(g, sp.accelerate, coeff=0.5, **kw)
+        sp.apply(g, ops["reset_rho!"])
+        sp.apply(g, ops["find_rho!"], self=True, **kw)
+        sp.apply(g, ops["find_pressure!"], **kw)
+        sp.apply(g, ops["reset_a!"])
+        sp.apply(g, ops["internal_force!"], **kw)
+        sp.apply(g, ops["accelerate!"], **kw)
         if k % every == 0:
             N.append(len(g))
             E.append(sp.energy(g, "collision", **kw))

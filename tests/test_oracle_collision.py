@@ -18,15 +18,15 @@ def run_collision(o, st, steps, every):
     N, E = [], []
     for k in range(steps + 1):
         # verlet_step! (tests/test_collision_2d.jl:106-116)
-        o.accelerate(coeff=0.5, **kw)
-        o.move(coeff=1.0, **kw)
+        o.kick(coeff=0.5, **kw)      # accelerate!, :92-94
+        o.drift(coeff=1.0, **kw)     # move!, :88-90
         o.create_cell_list()
         o.reset_rho()
         o.density_sum(True, **kw)
         o.eos(use_field=True, **kw)
         o.reset_a()
         o.pressure_force_rho0(**kw)
-        o.accelerate(coeff=0.5, **kw)
+        o.kick(coeff=0.5, **kw)
         if k % every == 0:
             N.append(o.n)
             E.append(o.energy("collision", **kw))

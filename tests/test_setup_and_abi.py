@@ -103,9 +103,27 @@ def test_system_desc_layout():
 
 def test_operator_registry():
     assert sp.OPERATORS["find_rho!"] is sp.find_rho and sp.find_rho.binary
-    assert not sp.OPERATORS["move!"].binary
-    with pytest.raises(NotImplementedError):
-        sp._resolve(lambda p, q, r: None)
+    # names the reference defines with different bodies need their program
+    for name in ("find_pressure!", "internal_force!", "move!", "accelerate!"):
+        with pytest.raises(KeyError):
+            sp._resolve(name)
+    cd = sp.program_operators("collapse_dry")
+    tc = sp.program_operators("test_collision_2d")
+    assert sp._resolve("collapse_dry:move!") is cd["move!"]
+    assert cd["move!"].id == N.SPH_OP_MOVE and cd["move!"].fixed == {"coeff": 0.5}
+    assert tc["move!"].id == N.SPH_OP_DRIFT and tc["move!"].fixed == {"coeff": 1.0}
+    assert cd["accelerate!"].id == N.SPH_OP_ACCELERATE and tc["accelerate!"].id == N.SPH_OP_KICK
+    assert cd["find_pressure!"].id == N.SPH_OP_FIND_PRESSURE_CONTINUITY
+    assert tc["find_pressure!"].id == N.SPH_OP_FIND_PRESSURE_RHO0
+    assert cd["internal_force!"].id == N.SPH_OP_INTERNAL_FORCE
+    assert tc["internal_force!"].id == N.SPH_OP_PRESSURE_FORCE_RHO0
+    # fixed parameters override the call's
+    assert sp._params(cd["move!"], dict(dt=1e-3, coeff=1.0)).coeff == 0.5
+    # a Python callable goes to the host fallback
+    assert sp._resolve(lambda p, q, r: None) is None
+    assert sp._arity(lambda p, q, r: None) == 3 and sp._arity(lambda p: None) == 1
+    with pytest.raises(KeyError):
+        sp._resolve("no_such_operator!")
 
 
 def test_host_kernels_match_reference_constants():
