@@ -23,13 +23,21 @@
  *   key geometry ............ closed-form integer tables (golden JSON)
  *   cell list / pairs ........ O(N^2) brute force, lattice neighbour counts,
  *                              exact-tie and duplicate-position cases
+ *   neighbour digest ......... brute-force pair list; mix64 against the
+ *                              splitmix64 finaliser's published outputs
  *   Wendland W / rDW ......... closed forms, quadrature normalisation,
  *                              rD = D/r (tests/test_kernels.jl:20-43)
- *   density sum .............. lattice sum computed independently in Python
- *   pressure force + AV ...... two-particle closed forms, momentum antisymmetry
- *   integrator ............... free-fall closed form
- *   energy ................... closed-form energy of hand-built states and the
+ *   density sums (rho, rho0).. lattice sums computed independently (fsum)
+ *   pressure forces + AV ..... two-particle closed forms (P/rho^2 and the
+ *                              rho0^2 form), momentum antisymmetry
+ *   continuity ............... two-particle closed form
+ *   EOS (three forms) ........ closed forms
+ *   move!/drift/kick ......... closed forms; free fall in closed form for the
+ *                              symplectic Euler step and for the leapfrog of
+ *                              the collapse_dry time loop
+ *   energy (three forms) ..... closed-form energy of hand-built states and the
  *                              energy-drift bound of tests/test_collision_2d.jl
+ *   momentum ................. exact sum (math.fsum)
  *   periodic systems ......... numpy minimum-image brute force
  *                              (tests/test_oracle_periodic.py)
  *   probe sums ............... brute force over all particles, an exact visit

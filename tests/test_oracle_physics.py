@@ -211,3 +211,192 @@ def test_eos():
     o.load(np.zeros((3, 3)), rho=np.array([990.0, 1000.0, 1010.0]))
     o.eos(c=50.0, rho0=1000.0)
     assert list(o.field("P")) == [2500.0 * -10.0, 0.0, 2500.0 * 10.0]
+
+
+# ---- pins of the remaining oracle functions -------------------------------
+
+def test_pressure_force_rho0_two_particles():
+    """tests/test_collision_2d.jl:75-78: a_p += -m rDW (P_p/rho0^2 +
+    P_q/rho0^2) x_pq, every particle a target (a wall too), written out by
+    hand for two particles; swapping which particle is the wall must not
+    change anything."""
+    m, h, dx, rho0 = 0.4, 0.1, 0.06, 1000.0
+    u = 1 - dx / h
+    rdw = -KC["dw2"] * u ** 3 / h ** 4
+    A = 1000.0 / rho0 ** 2 + 500.0 / rho0 ** 2
+    for types in ((0.0, 0.0), (0.0, 1.0)):
+        o = two_particles(dx, types=types)
+        o.pressure_force_rho0(m=m, rho0=rho0)
+        a = o.field("a")
+        assert a[0, 0] == pytest.approx(-m * rdw * A * (-dx), rel=1e-13)
+        assert a[1, 0] == pytest.approx(-m * rdw * A * dx, rel=1e-13)
+        assert a[0, 1] == 0.0 and a[1, 1] == 0.0
+    # rho does not enter (unlike internal_force!)
+    o = two_particles(dx, rho=(500.0, 2000.0))
+    o.pressure_force_rho0(m=m, rho0=rho0)
+    assert o.field("a")[0, 0] == pytest.approx(-m * rdw * A * (-dx), rel=1e-13)
+
+
+def test_eos_with_rho0_field_and_continuity_closed_forms():
+    """find_pressure! of tests/test_collision_2d.jl:71-73 (P = c^2 (rho -
+    rho0_p), the particle's own rho0) and the continuity form of
+    examples/collapse_dry.jl:123-127 (rho += Drho dt; Drho = 0; P = c^2
+    (rho - rho0))."""
+    o = O.OracleSystem(0.05, np.zeros(3), np.array([1.0, 1.0, 0.0]))
+    o.load(np.zeros((3, 3)), rho=np.array([990.0, 1000.0, 1010.0]))
+    o.field("rho0")[:] = [995.0, 1002.0, 1000.0]
+    o.eos(use_field=True, c=50.0, rho0=7.0)  # the constant is not used
+    assert list(o.field("P")) == [2500.0 * -5.0, 2500.0 * -2.0, 2500.0 * 10.0]
+    o.field("Drho")[:] = [100.0, -50.0, 0.0]
+    o.eos_continuity(c=50.0, rho0=1000.0, dt=0.01)
+    assert list(o.field("rho")) == [991.0, 999.5, 1010.0]
+    assert list(o.field("Drho")) == [0.0, 0.0, 0.0]
+    assert list(o.field("P")) == [2500.0 * -9.0, 2500.0 * -0.5, 2500.0 * 10.0]
+
+
+def test_energy_rho0_closed_form():
+    """WCSPH energy with the per-particle reference density: kinetic and
+    potential of examples/collapse_dry.jl:166-171 and the internal energy
+    m c^2 (ln(rho/rho0_p) + rho0_p/rho - 1) of P = c^2 (rho - rho0_p); the
+    constant rho0 must not enter.  Swapping rho and rho0_p of one particle
+    changes only that particle's internal term, which pins the argument
+    order."""
+    x = np.array([[0.1, 0.2, 0.0], [1.0, 2.0, 0.0]])
+    v = np.array([[1.0, 2.0, 0.0], [0.0, -1.0, 0.0]])
+    rho = np.array([1010.0, 990.0])
+    r0 = np.array([1003.0, 996.0])
+    m, c, g = 0.4, 50.0, np.array([0.0, -9.8, 0.0])
+    o = O.OracleSystem(0.05, np.zeros(3), np.array([4.0, 3.0, 0.0]))
+    o.load(x, v, rho, np.zeros(2), np.zeros(2))
+    o.field("rho0")[:] = r0
+
+    def e_of(rho_, r0_):
+        return math.fsum(0.5 * m * (v[i] @ v[i]) - m * (g @ x[i])
+                         + m * c * c * (math.log(rho_[i] / r0_[i]) + r0_[i] / rho_[i] - 1.0)
+                         for i in range(2))
+
+    assert o.energy("wcsph_rho0", m=m, c=c, rho0=123.0, g=g) == pytest.approx(e_of(rho, r0), rel=1e-14)
+    o.field("rho")[:] = [1003.0, 990.0]
+    o.field("rho0")[:] = [1010.0, 996.0]
+    e2 = e_of(np.array([1003.0, 990.0]), np.array([1010.0, 996.0]))
+    assert o.energy("wcsph_rho0", m=m, c=c, rho0=123.0, g=g) == pytest.approx(e2, rel=1e-14)
+    assert e2 != pytest.approx(e_of(rho, r0), rel=1e-9)
+
+
+def test_momentum_against_exact_sum():
+    """Sum of m v over all particles, against math.fsum (exact rounding)."""
+    rng = np.random.default_rng(11)
+    n = 5000
+    v = rng.normal(size=(n, 3)) * [3.0, 1e-3, 50.0]
+    o = O.OracleSystem(0.05, np.zeros(3), np.ones(3))
+    o.load(rng.random((n, 3)), v)
+    m = 0.731
+    got = o.momentum(m=m)
+    for d in range(3):
+        exact = math.fsum(m * float(a) for a in v[:, d])
+        assert got[d] == pytest.approx(exact, rel=1e-13, abs=1e-13 * m * np.abs(v[:, d]).sum())
+
+
+@pytest.mark.parametrize("dim,k", [(2, 4.8), (3, 2.4)])
+def test_density_rho0_lattice_sum(dim, k):
+    """find_rho0! (tests/test_collision_2d.jl:67-69) adds into rho0 what
+    find_rho! adds into rho: the interior lattice value, written out as a
+    math.fsum, and rho itself untouched."""
+    dr = 0.02
+    R = k * dr
+    m_ = 1000.0 * dr ** dim
+    n = 25 if dim == 2 else 13
+    x = lattice(dim, n, dr)
+    o = O.OracleSystem(R, np.zeros(3), x.max(0))
+    o.load(x, rho=np.full(len(x), 7.0))
+    o.create_cell_list()
+    o.density_sum_rho0(True, m=m_)
+    centre = np.argmin(((x - (n // 2) * dr) ** 2)[:, :dim].sum(1))
+    kk = int(math.ceil(k))
+    C = KC["w2"] if dim == 2 else KC["w3"]
+    terms = []
+    rng3 = range(-kk, kk + 1) if dim == 3 else (0,)
+    for i in range(-kk, kk + 1):
+        for j in range(-kk, kk + 1):
+            for l in rng3:
+                r = math.sqrt(i * i + j * j + l * l) * dr
+                if r <= R:
+                    q = r / R
+                    terms.append(m_ * C * (1 - q) ** 4 * (1 + 4 * q) / R ** dim)
+    assert o.field("rho0")[centre] == pytest.approx(math.fsum(terms), rel=1e-12)
+    assert np.all(o.field("rho") == 7.0)
+    # a second call accumulates (reset is the caller's)
+    o.density_sum_rho0(True, m=m_)
+    assert o.field("rho0")[centre] == pytest.approx(2 * math.fsum(terms), rel=1e-12)
+
+
+def test_move_drift_kick_closed_forms():
+    """move! of examples/collapse_dry.jl:148-153 zeroes Dv of every particle
+    and moves the fluid only; move! and accelerate! of
+    tests/test_collision_2d.jl:88-94 act on every particle, keep a, and add
+    no gravity."""
+    x = np.array([[0.5, 0.5, 0.0], [1.0, 1.0, 0.0]])
+    v = np.array([[1.0, -2.0, 0.0], [3.0, 4.0, 0.0]])
+    a = np.array([[10.0, 20.0, 0.0], [-5.0, 7.0, 0.0]])
+    typ = np.array([0.0, 1.0])
+    o = O.OracleSystem(0.1, np.zeros(3), np.array([2.0, 2.0, 0.0]))
+    o.load(x, v, np.full(2, 1000.0), np.zeros(2), typ)
+    o.field("a")[:] = a
+    o.move(dt=0.25, coeff=0.5)
+    assert np.array_equal(o.field("a"), np.zeros((2, 3)))
+    assert np.array_equal(o.field("x"), [[0.5 + 0.125, 0.5 - 0.25, 0.0], [1.0, 1.0, 0.0]])
+    o.field("x")[:] = x
+    o.field("a")[:] = a
+    o.drift(dt=0.25, coeff=1.0)
+    assert np.array_equal(o.field("x"), x + 0.25 * v) and np.array_equal(o.field("a"), a)
+    o.kick(dt=0.25, coeff=0.5, g=(0.0, -9.8, 0.0))
+    assert np.array_equal(o.field("v"), v + 0.125 * a)
+
+
+def test_collapse_dry_loop_free_fall():
+    """examples/collapse_dry.jl:202-211 on isolated fluid particles (no
+    neighbours, so Dv = 0 and Drho = 0): the half kicks and drifts form the
+    leapfrog, exact for a constant field: v_n = v0 + n dt g,
+    x_n = x0 + n dt v0 + g (n dt)^2 / 2; walls stay in place."""
+    x0 = np.array([[0.5, 2.0, 0.0], [3.0, 1.0, 0.0], [2.0, 2.5, 0.0]])
+    v0 = np.array([[0.3, 0.0, 0.0], [-0.2, 0.5, 0.0], [0.0, 0.0, 0.0]])
+    typ = np.array([0.0, 0.0, 1.0])
+    o = O.OracleSystem(0.01, np.zeros(3), np.array([4.0, 3.0, 0.0]))
+    o.load(x0, v0, np.full(3, 1000.0), np.zeros(3), typ)
+    g = np.array([0.0, -9.8, 0.0])
+    p = dict(m=0.1, rho0=1000.0, c=50.0, mu=1e-3, nu=1e-6, g=g, dt=1e-3)
+    o.create_cell_list()
+    o.internal_force(**p)
+    n = 40
+    assert o.step_collapse_dry(n, **p) == 0
+    t = n * 1e-3
+    assert np.allclose(o.field("v")[:2], v0[:2] + t * g, rtol=0, atol=1e-13)
+    assert np.allclose(o.field("x")[:2], x0[:2] + t * v0[:2] + 0.5 * g * t * t, rtol=0, atol=1e-13)
+    assert np.array_equal(o.field("x")[2], x0[2]) and np.array_equal(o.field("v")[2], v0[2])
+    assert np.all(o.field("rho") == 1000.0)
+
+
+def test_neighbour_digest_brute_force():
+    """Per-particle neighbour count and sum of mix64(q + 1), against the
+    O(N^2) brute-force pair list; mix64 pinned by the splitmix64 finaliser's
+    published first outputs from seed 0 (z = k * 0x9e3779b97f4a7c15)."""
+    first = [0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f]
+    M = (1 << 64) - 1
+    for k, ref in enumerate(first, 1):
+        assert O.mix64((k * 0x9E3779B97F4A7C15) & M) == ref
+    rng = np.random.default_rng(12)
+    for dim, n, h in ((2, 1500, 0.04), (3, 1500, 0.15)):
+        x = rng.random((n, 3))
+        if dim == 2:
+            x[:, 2] = 0.0
+        o = O.OracleSystem(h, np.zeros(3), np.array([1.0, 1.0, 1.0 if dim == 3 else 0.0]))
+        o.load(x)
+        o.create_cell_list()
+        cnt, hsh = o.neighbour_digest()
+        bp = O.brute_pairs(x, h)
+        c_ref = np.bincount(bp[:, 0], minlength=n)
+        h_ref = [0] * n
+        for p_, q_ in bp:
+            h_ref[p_] = (h_ref[p_] + O.mix64(int(q_) + 1)) & M
+        assert np.array_equal(cnt, c_ref)
+        assert [int(a) for a in hsh] == h_ref
