@@ -203,8 +203,14 @@ int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n);
 int sph_neighbour_digest(sph_system* s, int64_t* counts, uint64_t* hashes, int64_t n);
 
 /* nsteps fused steps of a scheme, device-resident, one host sync at the end.
+ * SPH_SCHEME_COLLAPSE_DRY is the time loop of examples/collapse_dry.jl:202-211.
+ * SPH_SCHEME_WCSPH_SE(_RHO0) is a symplectic Euler step assembled from the
+ * reference's operators (create_cell_list!, reset_rho!, find_rho! with self,
+ * find_pressure!, reset_a!, internal_force!, then a full-step fluid kick and
+ * drift); no reference program runs exactly this loop -- it is the
+ * density-summation scheme of the dam-break and Taylor-Green workloads.
  * *pairs (may be NULL) receives the accepted ordered pairs summed over the
- * density passes of all steps. */
+ * first pair pass of every step (density sum, or balance_of_mass!). */
 int sph_step(sph_system* s, int scheme, const sph_params* params, int nsteps,
              int64_t* pairs);
 
