@@ -185,6 +185,19 @@ int sph_reduce(sph_system* s, int what, const sph_params* params, double* out);
 int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const double* x,
                   int64_t np, double* out);
 
+/* sph_probe_sum without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * probe points and the sums are staged in a static device array of
+ * SPH_PROBE_NOALLOC_MAX * 7 doubles (3.5 MiB) that each GPU the library runs
+ * on holds until the process ends; np above SPH_PROBE_NOALLOC_MAX is
+ * rejected with SPH_EINVAL.  The call allocates nothing.  Calls that overlap
+ * on one GPU (several host threads) share the array and race.  Results are
+ * the bits of sph_probe_sum. */
+#define SPH_PROBE_NOALLOC_MAX 65536
+int sph_probe_sum_noalloc(sph_system* s, const sph_params* params, int flags, const double* x,
+                          int64_t np, double* out);
+
 /* Neighbour pairs of the current cell list as reference particle indices
  * (p, q), p != q, dist <= h: pairs[2k], pairs[2k+1].  The order is
  * unspecified.  *n receives the total; at most cap pairs are written. */

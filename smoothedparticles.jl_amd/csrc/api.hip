@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "sph_batch.h"
@@ -531,6 +532,7 @@ int new_handle(int precision, sph_system** out)
     s->use_masks = true;
     s->use_kept = true;
     int rc = 0;
+    if (hipGetDevice(&s->device) != hipSuccess) s->device = 0;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         s->stream = nullptr;
@@ -952,6 +954,59 @@ int probe_sum(sph_system* s, int slot, double m, int flags, const double* x, lon
     return 0;
 }
 
+// Probe sums without device allocation (sph_probe_sum_noalloc).  The probe
+// points and the results are staged in `scratch`, a file-scope device array
+// of SPH_PROBE_NOALLOC_MAX * 7 doubles that every GPU holds until the process
+// ends.  Its device address is taken once per device with
+// hipGetSymbolAddress (the array's name is a host-side stand-in) and kept in
+// scratch_ptr; calls that overlap on one device share the array and race.
+static __device__ __align__(256) double scratch[7 * SPH_PROBE_NOALLOC_MAX];
+
+static constexpr int MAX_DEVICES = 64;
+static double* scratch_ptr[MAX_DEVICES];
+static std::mutex scratch_mu;
+
+int scratch_of(int device, double** out)
+{
+    if (device < 0 || device >= MAX_DEVICES) return fail(SPH_EINVAL, "device id out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SPH_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = scratch_ptr[device];
+    return 0;
+}
+
+int probe_sum_noalloc(sph_system* s, double m, int flags, const double* x, long long np,
+                      double* out)
+{
+    if (np < 0) return fail(SPH_EINVAL, "probe count out of range");
+    if (np > SPH_PROBE_NOALLOC_MAX)
+        return fail(SPH_EINVAL, "sph_probe_sum_noalloc takes at most SPH_PROBE_NOALLOC_MAX (" +
+                                    std::to_string(SPH_PROBE_NOALLOC_MAX) + ") probes per call");
+    if (flags & ~SPH_PROBE_FLUID_ONLY) return fail(SPH_EINVAL, "unknown probe flags");
+    if (s->sys_host[0].g.periodic)
+        return fail(SPH_EUNSUP, "probe sums of periodic systems are not supported");
+    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    if (np == 0) return 0;
+    HIPC(hipSetDevice(s->device));
+    double* buf = nullptr;
+    int rc = scratch_of(s->device, &buf);
+    if (rc) return rc;
+    double* pin = buf;
+    double* pout = buf + 3 * np;
+    HIPC(hipMemcpyAsync(pin, x, sizeof(double) * 3 * np, hipMemcpyHostToDevice, s->stream));
+    enqueue_probe_sum(s, 0, pin, np, m, (flags & SPH_PROBE_FLUID_ONLY) ? 1 : 0, pout);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(out, pout, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
 // Neighbour pairs of the current cell lists; with only_slot >= 0, of that
 // system only (the pair kernels skip the tiles of the others).
 int neighbour_pairs(sph_system* s, int only_slot, int64_t* pairs, int64_t cap, int64_t* n)
@@ -1277,6 +1332,14 @@ int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const doub
     if (!s || !params || (np > 0 && (!x || !out))) return fail(SPH_EINVAL, "null argument");
     if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
     return probe_sum(s, 0, params->m, flags, x, np, out);
+}
+
+int sph_probe_sum_noalloc(sph_system* s, const sph_params* params, int flags, const double* x,
+                          int64_t np, double* out)
+{
+    if (!s || !params || (np > 0 && (!x || !out))) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return probe_sum_noalloc(s, params->m, flags, x, np, out);
 }
 
 int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)

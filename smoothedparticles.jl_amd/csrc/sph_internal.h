@@ -272,6 +272,7 @@ struct sph_system {
     long long cap;
     bool cells_valid;
     hipStream_t stream;
+    int device;          // the device the handle's memory and stream live on
 
     // systems
     std::vector<sph::SysDesc> sys_host;
