@@ -243,9 +243,14 @@ int sph_step_tests(sph_system* s, int64_t* density_tests);
 int sph_key_geometry(sph_system* s, int64_t* key_phase, int64_t* key_lim,
                      int64_t* key_max, int64_t* key_diff, int* n_diff);
 
-/* Targets per tile of the pair kernels (8, 16, 32 or 64; 0 = automatic).
- * sph_autotune times the density kernel for each tile size on the current
- * cell list and keeps the fastest; *best receives it. */
+/* Targets per tile of the pair kernels (8, 16, 32 or 64; 0 = automatic: the
+ * smallest power of two >= the mean occupancy of the non-empty cells, at
+ * most 64, chosen at each sph_build_cells).  The summation order of the
+ * pair sums, hence their last bits, depends on the tile size.
+ * sph_autotune: in SPH_MIXED mode it times the density and force passes for
+ * each tile size on the current cell list and keeps the fastest; in SPH_FP64
+ * mode it keeps the automatic choice, so the plain FP64 path gives the same
+ * bits for the same input on every run.  *best receives the tile size. */
 int sph_set_tile(sph_system* s, int targets_per_tile);
 int sph_autotune(sph_system* s, const sph_params* params, int* best);
 

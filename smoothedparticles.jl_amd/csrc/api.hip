@@ -1119,12 +1119,29 @@ int neighbour_digest(sph_system* s, int slot, int64_t* counts, uint64_t* hashes,
     return rc;
 }
 
-// Times the density and force passes for every tile size on the current
-// cell lists with a zero mass (both passes then add exactly zero to rho
-// and a, so the state is left as it was) and keeps the fastest.
+// Mixed precision: times the density and force passes for every tile size on
+// the current cell lists with a zero mass (both passes then add exactly zero
+// to rho and a, so the state is left as it was) and keeps the fastest.  The
+// mixed sums are folded per run in FP32 with 2Sum, so their bits depend on T
+// as well; the mixed path is stated by tolerance.  FP64: the occupancy rule.
 int autotune(sph_system* s, int* best)
 {
     if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    if (s->precision == SPH_FP64) {
+        // The FP64 sums are added per run and per slice of a tile, so their
+        // bits depend on the tile size: the plain FP64 path keeps the size
+        // the occupancy rule gives, never one picked by timing, and the same
+        // input gives the same bits on every run.
+        int rc = sph_set_tile(s, 0);
+        if (!rc && s->meta_host[META_NCELLS] > 0) {
+            const int T = choose_tile((double)s->n / s->meta_host[META_NCELLS]);
+            rc = sph_set_tile(s, T);
+            if (!rc) rc = sph_set_tile(s, 0);
+        }
+        if (rc) return rc;
+        if (best) *best = s->tile_T;
+        return 0;
+    }
     std::vector<Phys> saved;
     for (auto& d : s->sys_host) {
         saved.push_back(d.ph);

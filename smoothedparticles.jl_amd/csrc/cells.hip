@@ -14,7 +14,8 @@
 //   3. k_scatter  bucket every particle
 //   4. k_rank     order each bucket by descending reference index, the order
 //                 add_index! keeps (src/core.jl:26-41); deterministic
-//                 whatever order the atomics handed out
+//                 whatever order the atomics handed out (the bucket of
+//                 retired systems keeps its scatter order, never used)
 //   5. k_gather   permute every column into the new order
 //   6. removal    each system's removal bucket holds its removed particles in
 //                 descending reference index; the reference's swap-with-end
@@ -289,19 +290,56 @@ __global__ __launch_bounds__(256) void k_scatter(const int* meta, const int* __r
     tmp_ref[pos] = ref[i];
 }
 
+// Rank of each particle in its bucket: the number of bucket members with a
+// larger reference index (ties -- particles of different retired systems --
+// by scatter position), so the ranks are a permutation of the bucket.
+//   - the bucket of retired systems (dead_key) keeps its scatter order: its
+//     order is never used, and it can hold whole retired systems;
+//   - a block whose threads all lie in one bucket of more than RANK_LDS_MIN
+//     particles (a removal bucket after many particles left the box) counts
+//     against the bucket staged through LDS, 1024 indices at a time;
+//   - otherwise each thread scans its bucket (a cell: tens of particles).
+static constexpr int RANK_LDS_MIN = 512;
+
 __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __restrict__ cell_start,
                                               const int* __restrict__ tmp_idx,
                                               const int* __restrict__ tmp_key,
-                                              const int* __restrict__ tmp_ref, int* perm)
+                                              const int* __restrict__ tmp_ref, int* perm,
+                                              int dead_key)
 {
+    __shared__ int tile[1024];
     const int n = meta[META_NTOT];
-    const int u = blockIdx.x * 256 + threadIdx.x;
-    if (u >= n) return;
+    const int b0 = blockIdx.x * 256;
+    if (b0 >= n) return;  // whole block past the end (uniform)
+    const int u = b0 + threadIdx.x;
+    const bool in = u < n;
+    const int c0 = tmp_key[b0];
+    const int c1 = tmp_key[min(n - 1, b0 + 255)];
+    const int s0 = cell_start[c0], e0 = cell_start[c0 + 1];
+    if (c0 == c1 && c0 != dead_key && e0 - s0 > RANK_LDS_MIN) {
+        const int r = in ? tmp_ref[u] : 0;
+        int rank = 0;
+        for (int base = s0; base < e0; base += 1024) {
+            const int m = min(1024, e0 - base);
+            __syncthreads();
+            for (int j = threadIdx.x; j < m; j += 256) tile[j] = tmp_ref[base + j];
+            __syncthreads();
+            for (int j = 0; j < m; ++j) {
+                const int rv = tile[j];
+                rank += rv > r || (rv == r && base + j < u);
+            }
+        }
+        if (in) perm[s0 + rank] = tmp_idx[u];
+        return;
+    }
+    if (!in) return;
     const int c = tmp_key[u];
+    if (c == dead_key) {
+        perm[u] = tmp_idx[u];
+        return;
+    }
     const int s = cell_start[c], e = cell_start[c + 1];
     const int r = tmp_ref[u];
-    // ties (particles of different retired systems in the dead bucket) by
-    // scatter position, so the ranks are a permutation of the bucket
     int rank = 0;
 #pragma unroll 8
     for (int v = s; v < e; ++v) {
@@ -529,7 +567,7 @@ void enqueue_build(sph_system* s)
     k_scatter<<<nb, 256, 0, st>>>(s->meta, ic + I_KEY * cap, s->slot, ic + I_REF * cap,
                                   s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref);
     k_rank<<<nb, 256, 0, st>>>(s->meta, s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref,
-                               s->perm);
+                               s->perm, ncells + nsys);
     k_gather<<<nb, 256, 0, st>>>(s->meta, s->perm, c, s->cols[s->cur ^ 1], ic,
                                  s->icols[s->cur ^ 1], cap);
     s->cur ^= 1;

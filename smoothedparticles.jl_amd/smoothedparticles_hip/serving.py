@@ -11,10 +11,18 @@ stepping calls, so their work shares every launch with the others.
 Each batch is advanced in quanta of at most ``quantum`` steps, cut short so
 that no request runs past its step count; a request's trajectory therefore
 does not depend on what else is in the batch (the batch gives the same
-bits as a batch of one with the same tile size).
+bits as a batch of one with the same tile size).  The 2D and the 3D batch
+are stepped from two host threads, each on its own stream, so their
+kernels overlap on the device.
+
+Admission is by device footprint: a request is charged its particle count
+times the bytes per particle its batch holds, measured with
+sph_batch_footprint after every admission (the largest ratio seen; a
+starting value until the first measurement).
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -22,9 +30,9 @@ from . import Batch, states
 
 KINDS = ("dam_break_2d", "dam_break_3d", "taylor_green")
 
-# Device bytes per particle used for admission: particle columns, two
-# buffers, build scratch and pair masks, with the capacity doubling of the
-# particle arrays (measured footprints stay below these).
+# Starting value of the admission charge per particle, before a batch of the
+# dimension has been measured: particle columns, two buffers, build scratch
+# and pair masks, with the capacity doubling of the particle arrays.
 BYTES_PER_PARTICLE = {2: 1200, 3: 2400}
 
 
@@ -63,8 +71,9 @@ class Request:
     def n(self):
         return len(self.build().x)
 
-    def footprint(self):
-        return BYTES_PER_PARTICLE[self.dim] * self.n
+    def footprint(self, bytes_per_particle=None):
+        bpp = BYTES_PER_PARTICLE[self.dim] if bytes_per_particle is None else bytes_per_particle
+        return int(math.ceil(bpp * self.n))
 
     @property
     def latency(self):
@@ -103,11 +112,15 @@ def make_requests(count, rate, seed=0, n_lo=1000, n_hi=500000, steps_lo=200, ste
     return out
 
 
-def summary(st, b, sid):
-    """Result of a finished request: kinetic energy and momentum."""
+def summary(st, b, sid, fields=()):
+    """Result of a finished request: kinetic energy and momentum, plus the
+    final arrays of `fields`."""
     v = b.get(sid, "v")
-    return {"n": int(len(v)), "kinetic": 0.5 * st.m * float(np.sum(v * v)),
-            "momentum": (st.m * v.sum(axis=0)).tolist()}
+    out = {"n": int(len(v)), "kinetic": 0.5 * st.m * float(np.sum(v * v)),
+           "momentum": (st.m * v.sum(axis=0)).tolist()}
+    for f in fields:
+        out[f] = b.get(sid, f)
+    return out
 
 
 class Server:
@@ -118,20 +131,27 @@ class Server:
     quantum     steps per stepping call of a batch
     tiles       targets per tile for each dimension (fixed, so a request's
                 results do not depend on the batch it ran in)
+    keep        fields whose final arrays each result keeps
     """
 
-    def __init__(self, precision="mixed", budget=16 << 30, quantum=50, tiles=None):
+    def __init__(self, precision="mixed", budget=16 << 30, quantum=50, tiles=None, keep=()):
+        self.keep = tuple(keep)
         self.precision = precision
         self.budget = budget
         self.quantum = quantum
         self.tiles = tiles or {2: 32, 3: 64}
         self.batches = {}
         self.active = {2: [], 3: []}
+        self.bpp = dict(BYTES_PER_PARTICLE)  # admission charge per particle, per dimension
+        self.measured = {2: False, 3: False}
+        self.charge = {}                     # request id -> bytes charged
         self.in_use = 0
         self.peak_in_use = 0
+        self.peak_footprint = 0              # largest sph_batch_footprint total seen
         self.peak_active = 0
         self.pairs = 0
         self.particle_steps = 0
+        self.pool = ThreadPoolExecutor(max_workers=2)
 
     def _batch(self, dim):
         b = self.batches.get(dim)
@@ -141,8 +161,22 @@ class Server:
             self.batches[dim] = b
         return b
 
+    def cost(self, r):
+        return r.footprint(self.bpp[r.dim])
+
     def fits(self, r):
-        return self.in_use + r.footprint() <= self.budget
+        return self.in_use + self.cost(r) <= self.budget
+
+    def _measure(self, dim):
+        """Bytes per live particle the batch of `dim` holds (sph_batch_footprint)."""
+        b = self.batches[dim]
+        _, n = b.size()
+        if n > 0:
+            ratio = b.footprint() / n
+            self.bpp[dim] = ratio if not self.measured[dim] else max(self.bpp[dim], ratio)
+            self.measured[dim] = True
+        self.peak_footprint = max(self.peak_footprint,
+                                  sum(x.footprint() for x in self.batches.values()))
 
     def admit(self, r, now):
         st = r.build()
@@ -150,33 +184,47 @@ class Server:
         r.sid = b.add_state(st, periodic=r.periodic)
         r.admitted = now
         self.active[r.dim].append(r)
-        self.in_use += r.footprint()
+        self._measure(r.dim)
+        self.charge[r.rid] = self.cost(r)
+        self.in_use += self.charge[r.rid]
         self.peak_in_use = max(self.peak_in_use, self.in_use)
 
+    def _advance(self, dim):
+        """One quantum of the batch of `dim` (runs on a worker thread; the
+        library call releases the interpreter lock)."""
+        act = self.active[dim]
+        b = self.batches[dim]
+        b.init_rho0()  # the reference density of the requests that just joined
+        k = min(self.quantum, min(r.steps - r.done_steps for r in act))
+        b.step(k)
+        dp, fp = b.step_pairs()
+        return k, dp + fp
+
     def step_all(self, clock):
-        """Advance every non-empty batch by one quantum; returns the requests
-        that finished."""
+        """Advance every non-empty batch by one quantum, the 2D and 3D
+        batches concurrently; returns the requests that finished."""
         done = []
-        for dim in (2, 3):
+        dims = [d for d in (2, 3) if self.active[d]]
+        if len(dims) == 2:
+            futs = {d: self.pool.submit(self._advance, d) for d in dims}
+            res = {d: f.result() for d, f in futs.items()}
+        else:
+            res = {d: self._advance(d) for d in dims}
+        for dim in dims:
             act = self.active[dim]
-            if not act:
-                continue
             b = self.batches[dim]
-            b.init_rho0()  # the reference density of the requests that just joined
-            k = min(self.quantum, min(r.steps - r.done_steps for r in act))
-            b.step(k)
-            dp, fp = b.step_pairs()
-            self.pairs += dp + fp
+            k, pairs = res[dim]
+            self.pairs += pairs
             self.particle_steps += k * sum(r.n for r in act)
             now = clock()
             keep = []
             for r in act:
                 r.done_steps += k
                 if r.done_steps >= r.steps:
-                    r.result = summary(r.state, b, r.sid)
+                    r.result = summary(r.state, b, r.sid, self.keep)
                     r.finished = now
                     b.retire(r.sid)
-                    self.in_use -= r.footprint()
+                    self.in_use -= self.charge.pop(r.rid)
                     r.state = None  # host arrays are no longer needed
                     done.append(r)
                 else:
@@ -224,6 +272,7 @@ class Server:
         return finished
 
     def close(self):
+        self.pool.shutdown()
         for b in self.batches.values():
             b.close()
         self.batches = {}

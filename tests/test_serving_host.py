@@ -26,3 +26,4 @@ def test_state_size_follows_request(kind, n):
     assert 0.75 * n <= st.n_fluid <= 1.25 * n
     r = serving.Request(0, kind, n, 100, 0.0)
     assert r.footprint() == serving.BYTES_PER_PARTICLE[r.dim] * len(st.x)
+    assert r.footprint(1500.5) == int(np.ceil(1500.5 * len(st.x)))
