@@ -185,6 +185,23 @@ int sph_reduce(sph_system* s, int what, const sph_params* params, double* out);
 int sph_probe_sum(sph_system* s, const sph_params* params, int flags, const double* x,
                   int64_t np, double* out);
 
+/* SmoothedParticles.sum(sys, func, p) (src/core.jl:271-291) for every
+ * particle p: out[p] (reference order, n = sph_count) receives the sum of
+ * func(p, q, r) over the particles q the reference visits from p (the cells
+ * find_key(p.x) + key_diff in that order, each in its stored order) whose
+ * r = norm(p.x - q.x) does not exceed h -- q == p included (r = 0), unlike
+ * apply_binary!.  Sums are FP64 in the reference's order in either precision
+ * mode.  func is a registered function:
+ *   SPH_SUM_LJ_WALL  LJ_potential, examples/collapse_symplectic.jl:144-151;
+ *                    fparams = {m, dr_wall, E_wall, eps}: for q a wall
+ *                    particle (type 1), p fluid and r < dr_wall,
+ *                    m E_wall (0.5 s^2 - 0.25 s^4 - 0.25), s = dr_wall/(r+eps),
+ *                    with s^4 formed as (s^2)^2; else 0.
+ * Needs a current cell list; periodic systems: SPH_EUNSUP. */
+#define SPH_SUM_LJ_WALL 1
+int sph_particle_sum(sph_system* s, int func, const double* fparams, int nfparams, double* out,
+                     int64_t n);
+
 /* sph_probe_sum without device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
@@ -264,6 +281,12 @@ int sph_autotune(sph_system* s, const sph_params* params, int* best);
  * candidate list again.  Either setting gives the same bits. */
 #define SPH_OPTION_PAIR_MASKS 0
 #define SPH_OPTION_KEPT_LISTS 1
+/* SPH_OPTION_GRAPH (default 0): sph_step / sph_batch_step capture the
+ * kernels of one or two steps into a hipGraph once per call and replay it
+ * for the call's steps, instead of enqueueing every kernel of every step
+ * (about 25 launches per step).  Same bits; kernel profiling is off while
+ * the graph replays. */
+#define SPH_OPTION_GRAPH 2
 int sph_set_option(sph_system* s, int option, int value);
 
 /* Diagnostics (no counterpart in the reference): candidate particles the

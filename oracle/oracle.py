@@ -72,6 +72,8 @@ def lib():
             "orc_neighbour_digest": (ctypes.c_int64, [ctypes.c_void_p, _pi,
                                                       ctypes.POINTER(ctypes.c_uint64)]),
             "orc_mix64": (ctypes.c_uint64, [ctypes.c_uint64]),
+            "orc_particle_sum_lj": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double, _pd]),
             "orc_energy": (ctypes.c_double, [ctypes.c_void_p, _pp, ctypes.c_int]),
             "orc_momentum": (None, [ctypes.c_void_p, _pp, _pd]),
             "orc_step_wcsph": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int, ctypes.c_int]),
@@ -201,6 +203,16 @@ class OracleSystem:
         if rc != 0:
             raise RuntimeError("orc_probe_sum: no current cell list or periodic system")
         return out
+
+    def particle_sum_lj(self, m, dr_wall, E_wall, eps=1e-16):
+        """SmoothedParticles.sum(sys, LJ_potential, p) for every particle p
+        (src/core.jl:271-291, examples/collapse_symplectic.jl:144-151)."""
+        out = np.zeros(max(self.n, 1))
+        rc = lib().orc_particle_sum_lj(self._s, float(m), float(dr_wall), float(E_wall),
+                                       float(eps), out.ctypes.data_as(_pd))
+        if rc != 0:
+            raise RuntimeError("orc_particle_sum_lj: no current cell list or periodic system")
+        return out[: self.n]
 
     def density_sum(self, self_term=True, **kw):
         p = params(**kw)

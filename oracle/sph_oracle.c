@@ -609,6 +609,45 @@ int orc_probe_sum(const orc_sys* s, double m, int fluid_only, const double* x, i
     return 0;
 }
 
+/* SmoothedParticles.sum(sys, func, p), src/core.jl:271-291, for every
+ * particle p, with func = LJ_potential of examples/collapse_symplectic.jl:
+ * 144-151: out[p] = sum over the q visited from find_key(p.x) + key_diff
+ * (in that order, each cell in its stored order) with r = norm(p.x - q.x)
+ * not above h -- q == p included -- of m E_wall (0.5 s^2 - 0.25 s^4 - 0.25),
+ * s = dr_wall/(r + eps), when q is a wall particle (type 1), p a fluid
+ * particle and r < dr_wall, else 0.  s^4 is (s^2)^2 here; Julia's s^4 is
+ * not restated at the ulp (parity unpinned there).  Returns -1 without a
+ * current cell list or for a periodic system. */
+int orc_particle_sum_lj(const orc_sys* s, double m, double dr_wall, double E_wall, double eps,
+                        double* out)
+{
+    if (!s->cells_valid || s->periodic) return -1;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t p = 0; p < s->n; ++p) {
+        const double* xp = s->x + 3 * p;
+        double acc = 0.0;
+        int64_t key = find_key(s, xp);
+        for (int t = 0; t < s->n_diff; ++t) {
+            int64_t nk = key + s->key_diff[t];
+            if (!(1 <= nk && nk <= s->key_max)) continue;
+            for (int64_t e = s->cell_start[nk - 1]; e < s->cell_start[nk]; ++e) {
+                int64_t q = s->cell_idx[e];
+                double r = dist3(xp, s->x + 3 * q);
+                if (r > s->h) continue;
+                double term = 0.0;
+                if (s->type[q] == 1.0 && s->type[p] == 0.0 && r < dr_wall) {
+                    double sq = dr_wall / (r + eps);
+                    double s2 = sq * sq;
+                    term = (m * E_wall) * ((0.5 * s2 - 0.25 * (s2 * s2)) - 0.25);
+                }
+                acc = acc + term;
+            }
+        }
+        out[p] = acc;
+    }
+    return 0;
+}
+
 /* O(N^2) brute force with the same FP64 predicate and no cells: all ordered
  * pairs p != q with dist <= h, p ascending then q ascending. */
 int64_t orc_brute_pairs(const double* x, int64_t n, double h, int64_t* out, int64_t cap)

@@ -30,6 +30,8 @@ void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long lo
 void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,
                        double* out_dev);
 void enqueue_key_out(sph_system* s, long long off, long long n, int cell_base, double* out_dev);
+void enqueue_particle_sum_lj(sph_system* s, int sys, long long off, long long n,
+                             const double* fp, double* out_dev);
 void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int ncomp,
                       const double* in_dev);
 void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const double* in_dev,
@@ -744,6 +746,72 @@ int cd_step(sph_system* s, long long tiles_bound)
     return 0;
 }
 
+// One step of a density-summation scheme: build; density pass (stores the
+// masks); EOS with P/rho^2; force pass (loads them); kick and drift.
+int ds_step(sph_system* s, int eos, long long tiles_bound)
+{
+    int rc;
+    hipEvent_t ev[2];
+    prof_begin(s, ev);
+    enqueue_build(s);
+    prof_end(s, SPH_KERNEL_BUILD, ev);
+    if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
+                       MASK_TEST_STORE, 0, s->pair_counter + 2)))
+        return rc;
+    prof_begin(s, ev);
+    enqueue_unary(s, eos, s->prho2);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
+    if ((rc = run_pair(s, P_FORCE, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE,
+                       MASK_LOAD)))
+        return rc;
+    prof_begin(s, ev);
+    enqueue_kick_drift(s);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
+    HIPC(hipGetLastError());
+    return 0;
+}
+
+int one_step(sph_system* s, int scheme, int eos, long long tiles_bound)
+{
+    return scheme == SPH_SCHEME_COLLAPSE_DRY ? cd_step(s, tiles_bound)
+                                             : ds_step(s, eos, tiles_bound);
+}
+
+// With SPH_OPTION_GRAPH the steps of a call are replayed from a hipGraph:
+// every kernel of a step reads its sizes from device counters, and nothing
+// the host enqueues changes while the call runs, so one capture serves the
+// whole call.  A build swaps the two column buffers, so the graph holds
+// steps enough for an even number of builds (two density-summation steps,
+// one collapse_dry step) and leaves the buffers where it found them.  The
+// pair counters are cleared before the replays, outside the graph.
+int run_graph(sph_system* s, int scheme, int eos, long long tiles_bound, int per, int reps)
+{
+    const bool prof = s->prof;
+    s->prof = false;  // no event records inside the capture
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = 0;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        s->prof = prof;
+        return fail(SPH_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    for (int k = 0; k < per && !rc; ++k) rc = one_step(s, scheme, eos, tiles_bound);
+    e = hipStreamEndCapture(s->stream, &graph);
+    s->prof = prof;
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SPH_EHIP, std::string("step graph: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int run_steps(sph_system* s, int scheme, int nsteps)
 {
     if (scheme != SPH_SCHEME_WCSPH_SE && scheme != SPH_SCHEME_WCSPH_SE_RHO0 &&
@@ -759,29 +827,14 @@ int run_steps(sph_system* s, int scheme, int nsteps)
     // Tiles are taken from a work queue: the grid needs no tile count.
     const long long tiles_bound = s->tile_cap;
     HIPC(hipMemsetAsync(s->pair_counter, 0, 3 * sizeof(unsigned long long), s->stream));
-    for (int k = 0; k < nsteps; ++k) {
-        if (scheme == SPH_SCHEME_COLLAPSE_DRY) {
-            if ((rc = cd_step(s, tiles_bound))) return rc;
-            continue;
-        }
-        hipEvent_t ev[2];
-        prof_begin(s, ev);
-        enqueue_build(s);
-        prof_end(s, SPH_KERNEL_BUILD, ev);
-        if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
-                           MASK_TEST_STORE, 0, s->pair_counter + 2)))
-            return rc;
-        prof_begin(s, ev);
-        enqueue_unary(s, eos, s->prho2);
-        prof_end(s, SPH_KERNEL_UNARY, ev);
-        if ((rc = run_pair(s, P_FORCE, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE,
-                           MASK_LOAD)))
-            return rc;
-        prof_begin(s, ev);
-        enqueue_kick_drift(s);
-        prof_end(s, SPH_KERNEL_UNARY, ev);
-        HIPC(hipGetLastError());
+    int k = 0;
+    const int per = scheme == SPH_SCHEME_COLLAPSE_DRY ? 1 : 2;
+    if (s->use_graph && nsteps / per >= 2) {
+        if ((rc = run_graph(s, scheme, eos, tiles_bound, per, nsteps / per))) return rc;
+        k = nsteps / per * per;
     }
+    for (; k < nsteps; ++k)
+        if ((rc = one_step(s, scheme, eos, tiles_bound))) return rc;
     HIPC(hipMemcpyAsync(s->step_pairs, s->pair_counter, 3 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s->stream));
     if ((rc = sync_meta(s))) return rc;
@@ -951,6 +1004,26 @@ int probe_sum(sph_system* s, int slot, double m, int flags, const double* x, lon
     const hipError_t ef = hipFree(buf);
     if (e != hipSuccess) return fail(SPH_EHIP, std::string("probe sum: ") + hipGetErrorString(e));
     if (ef != hipSuccess) return fail(SPH_EHIP, std::string("hipFree: ") + hipGetErrorString(ef));
+    return 0;
+}
+
+// SmoothedParticles.sum(sys, func, p) for every particle p of system `slot`
+// (src/core.jl:271-291) with a registered func; out[reference index].  The
+// sums are staged in the field-transfer buffer.
+int particle_sum(sph_system* s, int slot, int func, const double* fp, int nfp, double* out,
+                 int64_t n)
+{
+    if (func != SPH_SUM_LJ_WALL) return fail(SPH_EUNSUP, "unknown particle-sum function");
+    if (nfp != 4 || !fp) return fail(SPH_EINVAL, "SPH_SUM_LJ_WALL takes 4 parameters");
+    if (s->sys_host[slot].g.periodic)
+        return fail(SPH_EUNSUP, "particle sums of periodic systems are not supported");
+    if (!s->cells_valid) return fail(SPH_ESTATE, "cell list is not current");
+    if (n != s->sys_n[slot]) return fail(SPH_EINVAL, "n must equal the particle count");
+    if (n == 0) return 0;
+    enqueue_particle_sum_lj(s, slot, s->sys_off[slot], n, fp, s->stage);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(out, s->stage, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+    HIPC(hipStreamSynchronize(s->stream));
     return 0;
 }
 
@@ -1359,6 +1432,14 @@ int sph_probe_sum_noalloc(sph_system* s, const sph_params* params, int flags, co
     return probe_sum_noalloc(s, params->m, flags, x, np, out);
 }
 
+int sph_particle_sum(sph_system* s, int func, const double* fparams, int nfparams, double* out,
+                     int64_t n)
+{
+    if (!s || (n > 0 && !out)) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return particle_sum(s, 0, func, fparams, nfparams, out, n);
+}
+
 int sph_neighbour_pairs(sph_system* s, int64_t* pairs, int64_t cap, int64_t* n)
 {
     if (!s || !n) return fail(SPH_EINVAL, "null argument");
@@ -1473,6 +1554,9 @@ int sph_set_option(sph_system* s, int option, int value)
     case SPH_OPTION_KEPT_LISTS:
         s->use_kept = value != 0;
         s->masks_valid = false;
+        return 0;
+    case SPH_OPTION_GRAPH:
+        s->use_graph = value != 0;
         return 0;
     }
     return fail(SPH_EINVAL, "unknown option");

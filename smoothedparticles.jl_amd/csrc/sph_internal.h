@@ -324,6 +324,7 @@ struct sph_system {
     bool masks_valid;                // masks match the cell list and positions
     bool use_masks;                  // option: reuse masks across passes
     bool use_kept;                   // option: kept-candidate lists with the masks
+    bool use_graph;                  // option: replay the steps of a call from a hipGraph
     double* prho2;       // cap
     double* stage;       // 9 * cap (field IO)
     double* red_part;    // reduction partials

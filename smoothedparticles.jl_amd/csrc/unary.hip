@@ -302,54 +302,104 @@ __device__ inline double readlane_f64(double v, int j)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// The terms of the accepted candidates in ring slots [done, done + cnt) of
-// one probe, cnt <= 64, added to a0..a3 in slot order.
-__device__ inline void probe_flush(const Geo& g, const double* __restrict__ cols, long long cap,
-                                   double m, int fluid_only, const int* qe, const double* qs,
-                                   int done, int cnt, int lane, double& a0, double& a1,
-                                   double& a2, double& a3)
-{
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    if (lane < cnt) {
-        const int slot = (done + lane) & 127;
-        const int e = qe[slot];
-        const double r = sqrt(qs[slot]);
+// Term policies of the ordered sums.  terms() forms the NA terms of one
+// accepted candidate e at squared distance s; begin() reads what the sum
+// needs of its own point when the point is a particle.
+//
+// ProbeTerms: the interpolation closures of examples/cavity_flow.jl:166-172,
+// (phi m) W(h, r) and ((phi m) v_k) W(h, r), phi = Float64(type == FLUID)
+// with fluid_only, else 1.
+struct ProbeTerms {
+    static constexpr int NA = 4;
+    double m;
+    int fluid_only;
+    __device__ void begin(const double*, long long, int) {}
+    __device__ void terms(const Geo& g, const double* __restrict__ cols, long long cap, int e,
+                          double s, double* t) const
+    {
+        const double r = sqrt(s);
         const double phi = fluid_only ? (UCOL(cols, cap, C_TYPE, e) == 0.0 ? 1.0 : 0.0) : 1.0;
         const double w = phi * m;
         const double k = probe_w(g, r);
-        t0 = w * k;
-        t1 = (w * UCOL(cols, cap, C_VX, e)) * k;
-        t2 = (w * UCOL(cols, cap, C_VY, e)) * k;
-        t3 = (w * UCOL(cols, cap, C_VZ, e)) * k;
+        t[0] = w * k;
+        t[1] = (w * UCOL(cols, cap, C_VX, e)) * k;
+        t[2] = (w * UCOL(cols, cap, C_VY, e)) * k;
+        t[3] = (w * UCOL(cols, cap, C_VZ, e)) * k;
     }
-    for (int j = 0; j < cnt; ++j) {
-        a0 = a0 + readlane_f64(t0, j);
-        a1 = a1 + readlane_f64(t1, j);
-        a2 = a2 + readlane_f64(t2, j);
-        a3 = a3 + readlane_f64(t3, j);
+};
+
+// LJWallTerms: LJ_potential(p, q, r) of examples/collapse_symplectic.jl:144-151,
+// m E_wall (0.5 s^2 - 0.25 s^4 - 0.25) with s = dr_wall / (r + eps) when q is
+// a wall particle, p a fluid particle and r < dr_wall, else 0.  s^4 is
+// formed as (s^2)^2.
+struct LJWallTerms {
+    static constexpr int NA = 1;
+    double m, dr_wall, E_wall, eps;
+    bool p_fluid;
+    __device__ void begin(const double* __restrict__ cols, long long cap, int k)
+    {
+        p_fluid = UCOL(cols, cap, C_TYPE, k) == 0.0;
     }
+    __device__ void terms(const Geo&, const double* __restrict__ cols, long long cap, int e,
+                          double s, double* t) const
+    {
+        const double r = sqrt(s);
+        t[0] = 0.0;
+        if (p_fluid && UCOL(cols, cap, C_TYPE, e) == 1.0 && r < dr_wall) {
+            const double q = dr_wall / (r + eps);
+            const double q2 = q * q;
+            t[0] = (m * E_wall) * ((0.5 * q2 - 0.25 * (q2 * q2)) - 0.25);
+        }
+    }
+};
+
+// The terms of the accepted candidates in ring slots [done, done + cnt) of
+// one point, cnt <= 64, added to a[] in slot order.
+template <class Pol>
+__device__ inline void sum_flush(const Geo& g, const double* __restrict__ cols, long long cap,
+                                 const Pol& pol, const int* qe, const double* qs, int done,
+                                 int cnt, int lane, double* a)
+{
+    double t[Pol::NA];
+    for (int j = 0; j < Pol::NA; ++j) t[j] = 0.0;
+    if (lane < cnt) {
+        const int slot = (done + lane) & 127;
+        pol.terms(g, cols, cap, qe[slot], qs[slot], t);
+    }
+    for (int j = 0; j < cnt; ++j)
+        for (int v = 0; v < Pol::NA; ++v) a[v] = a[v] + readlane_f64(t[v], j);
 }
 
-// out[4i] = sum of (phi m) W(h, r), out[4i+1+k] = sum of ((phi m) v_k) W(h, r)
-// over the particles visited from probe i whose r > h is false (a NaN r is
-// kept, as in the reference); phi = Float64(type == FLUID) with fluid_only,
-// else 1.  One probe per block of one wave.  The test r > h is evaluated as
-// s > s_thr (s_thr = max{s : RN(sqrt(s)) <= h}, and a NaN s passes both);
-// accepted candidates are appended in order to a 128-slot ring, and the
-// square root, W and the terms are formed 64 accepted candidates at a time.
-__global__ __launch_bounds__(64) void k_probe_sum(Geo g, int cell_base,
-                                                  const int* __restrict__ cell_start,
-                                                  const double* __restrict__ cols, long long cap,
-                                                  const double* __restrict__ probes, int np,
-                                                  double m, int fluid_only,
-                                                  double* __restrict__ out)
+// SmoothedParticles.sum in the reference's order, one point per block of one
+// wave: point i is probes[3i..3i+2] (the point form, src/core.jl:240-258;
+// out[NA i + v]) or, with probes == nullptr, the particle at device slot
+// off + i (the particle form, :271-291, q == p included; out[NA ref + v] at
+// the particle's reference index).  Candidates whose r > h is false are kept
+// (a NaN r is kept, as in the reference); the test is evaluated as s > s_thr
+// (s_thr = max{s : RN(sqrt(s)) <= h}, and a NaN s passes both).  Accepted
+// candidates are appended in order to a 128-slot ring and their terms formed
+// 64 at a time.
+template <class Pol>
+__global__ __launch_bounds__(64) void k_ordered_sum(Geo g, int cell_base,
+                                                    const int* __restrict__ cell_start,
+                                                    const double* __restrict__ cols, long long cap,
+                                                    const double* __restrict__ probes,
+                                                    const int* __restrict__ ref, int off, int np,
+                                                    Pol pol, double* __restrict__ out)
 {
     __shared__ int qe[128];
     __shared__ double qs[128];
     const int i = blockIdx.x;
     const int lane = (int)threadIdx.x;
     if (i >= np) return;  // block-uniform
-    const double x = probes[3 * i], y = probes[3 * i + 1], z = probes[3 * i + 2];
+    double x, y, z;
+    if (probes) {
+        x = probes[3 * i]; y = probes[3 * i + 1]; z = probes[3 * i + 2];
+    } else {
+        const int k = off + i;
+        x = UCOL(cols, cap, C_X, k); y = UCOL(cols, cap, C_Y, k); z = UCOL(cols, cap, C_Z, k);
+        pol.begin(cols, cap, k);
+    }
     const long long key = probe_key(g, x, y, z);
     const long long L1 = g.L1, L12 = (long long)g.L1 * g.L2;
     const int kr = g.L3 == 1 ? 0 : 1;
@@ -374,7 +424,8 @@ __global__ __launch_bounds__(64) void k_probe_sum(Geo g, int cell_base,
         if (lane >= o) incl += v;
     }
     const int total = __shfl(incl, 63);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    double a[Pol::NA];
+    for (int v = 0; v < Pol::NA; ++v) a[v] = 0.0;
     int fill = 0, done = 0;  // ring positions, uniform
     int t0c = 0;             // first cell holding candidate `base`, uniform
     for (int base = 0; base < total; base += 64) {
@@ -408,17 +459,15 @@ __global__ __launch_bounds__(64) void k_probe_sum(Geo g, int cell_base,
         fill += (int)__popcll(acc);
         __syncthreads();
         if (fill - done >= 64) {
-            probe_flush(g, cols, cap, m, fluid_only, qe, qs, done, 64, lane, a0, a1, a2, a3);
+            sum_flush(g, cols, cap, pol, qe, qs, done, 64, lane, a);
             done += 64;
             __syncthreads();  // the slots read here are rewritten by the next appends
         }
     }
-    probe_flush(g, cols, cap, m, fluid_only, qe, qs, done, fill - done, lane, a0, a1, a2, a3);
+    sum_flush(g, cols, cap, pol, qe, qs, done, fill - done, lane, a);
     if (lane == 0) {
-        out[4 * i] = a0;
-        out[4 * i + 1] = a1;
-        out[4 * i + 2] = a2;
-        out[4 * i + 3] = a3;
+        const long long o = probes ? (long long)i : (long long)ref[off + i];
+        for (int v = 0; v < Pol::NA; ++v) out[Pol::NA * o + v] = a[v];
     }
 }
 
@@ -549,9 +598,23 @@ void enqueue_probe_sum(sph_system* s, int sys, const double* probes_dev, long lo
 {
     if (np <= 0) return;
     const SysDesc& d = s->sys_host[sys];
-    k_probe_sum<<<(unsigned)np, 64, 0, s->stream>>>(d.g, d.cell_base, s->cell_start,
-                                                 s->cols[s->cur], s->cap, probes_dev, (int)np, m,
-                                                 fluid_only, out_dev);
+    ProbeTerms pol{m, fluid_only};
+    k_ordered_sum<ProbeTerms><<<(unsigned)np, 64, 0, s->stream>>>(
+        d.g, d.cell_base, s->cell_start, s->cols[s->cur], s->cap, probes_dev, nullptr, 0, (int)np,
+        pol, out_dev);
+}
+
+// SmoothedParticles.sum(sys, LJ_potential, p) for every particle of system
+// `sys` (device slots [off, off + n)); out_dev[refidx] receives the sums.
+void enqueue_particle_sum_lj(sph_system* s, int sys, long long off, long long n,
+                             const double* fp, double* out_dev)
+{
+    if (n <= 0) return;
+    const SysDesc& d = s->sys_host[sys];
+    LJWallTerms pol{fp[0], fp[1], fp[2], fp[3], false};
+    k_ordered_sum<LJWallTerms><<<(unsigned)n, 64, 0, s->stream>>>(
+        d.g, d.cell_base, s->cell_start, s->cols[s->cur], s->cap, nullptr,
+        s->icols[s->cur] + I_REF * s->cap, (int)off, (int)n, pol, out_dev);
 }
 
 void enqueue_field_out(sph_system* s, long long off, long long n, int col0, int ncomp,

@@ -40,7 +40,7 @@ __all__ = [
     "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
     "probe_sums", "assemble_vector", "assemble_matrix", "drift", "kick", "PROGRAMS",
     "program_operators", "host_apply", "Particle", "neighbour_digest", "cell_keys",
-    "visit_order",
+    "visit_order", "particle_sums_lj_wall", "collapse_dry_init",
 ]
 
 FLUID = 0.0
@@ -277,6 +277,11 @@ class ParticleSystem:
         pass and gather only those in the passes that read its masks
         (default on)."""
         check(lib().sph_set_option(self._h, N.SPH_OPTION_KEPT_LISTS, 1 if enable else 0))
+
+    def set_graph(self, enable):
+        """Replay the fused steps of a step() call from a hipGraph (default
+        off; same bits)."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_GRAPH, 1 if enable else 0))
 
     def step_pairs(self):
         """(density-pass pairs, force-pass pairs) of the last step() call."""
@@ -594,6 +599,19 @@ def probe_sums(sys, x, fluid_only=False, **params):
     return out
 
 
+def particle_sums_lj_wall(sys, m, dr_wall, E_wall, eps=1e-16):
+    """SmoothedParticles.sum(sys, LJ_potential, p) (src/core.jl:271-291) for
+    every particle p, with LJ_potential of
+    examples/collapse_symplectic.jl:144-151: the wall-potential energy of
+    each particle (reference order).  q == p is visited too, as in the
+    reference.  Needs a current cell list."""
+    n = len(sys)
+    out = np.empty(max(n, 1), dtype=np.float64)
+    fp = np.array([m, dr_wall, E_wall, eps], dtype=np.float64)
+    check(lib().sph_particle_sum(sys._h, N.SPH_SUM_LJ_WALL, _as_ptr(fp), 4, _as_ptr(out), n))
+    return out[:n]
+
+
 def _fields(sys):
     return {k: sys.get(k) for k in N.FIELDS}
 
@@ -834,6 +852,11 @@ class Batch:
         """Gather only the candidates each tile's cull kept in the passes that
         read the masks (default on)."""
         check(lib().sph_set_option(self._h, N.SPH_OPTION_KEPT_LISTS, 1 if enable else 0))
+
+    def set_graph(self, enable):
+        """Replay the fused steps of a step() call from a hipGraph (default
+        off; same bits)."""
+        check(lib().sph_set_option(self._h, N.SPH_OPTION_GRAPH, 1 if enable else 0))
 
     def autotune(self):
         best = ctypes.c_int()

@@ -53,8 +53,10 @@ SPH_SCHEME_COLLAPSE_DRY = 2
 
 SPH_OPTION_PAIR_MASKS = 0
 SPH_OPTION_KEPT_LISTS = 1
+SPH_OPTION_GRAPH = 2
 SPH_PROBE_FLUID_ONLY = 1
 SPH_PROBE_NOALLOC_MAX = 65536
+SPH_SUM_LJ_WALL = 1
 
 SPH_KERNEL_BUILD = 0
 SPH_KERNEL_DENSITY = 1
@@ -113,6 +115,7 @@ SIGNATURES = {
     "sph_probe_sum": (_i, [_p, ctypes.POINTER(sph_params), _i, _pd, _i64, _pd]),
     "sph_batch_probe_sum": (_i, [_p, _i, _i, _pd, _i64, _pd]),
     "sph_probe_sum_noalloc": (_i, [_p, ctypes.POINTER(sph_params), _i, _pd, _i64, _pd]),
+    "sph_particle_sum": (_i, [_p, _i, _pd, _i, _pd, _i64]),
     "sph_step": (_i, [_p, _i, ctypes.POINTER(sph_params), _i, _pi64]),
     "sph_step_pairs": (_i, [_p, _pi64, _pi64]),
     "sph_step_tests": (_i, [_p, _pi64]),

@@ -132,10 +132,13 @@ class Server:
     tiles       targets per tile for each dimension (fixed, so a request's
                 results do not depend on the batch it ran in)
     keep        fields whose final arrays each result keeps
+    graph       replay each stepping call's steps from a hipGraph
     """
 
-    def __init__(self, precision="mixed", budget=16 << 30, quantum=50, tiles=None, keep=()):
+    def __init__(self, precision="mixed", budget=16 << 30, quantum=50, tiles=None, keep=(),
+                 graph=True):
         self.keep = tuple(keep)
+        self.graph = graph
         self.precision = precision
         self.budget = budget
         self.quantum = quantum
@@ -158,6 +161,7 @@ class Server:
         if b is None:
             b = Batch(self.precision)
             b.set_tile(self.tiles[dim])
+            b.set_graph(self.graph)
             self.batches[dim] = b
         return b
 

@@ -153,3 +153,33 @@ def test_probe_sums_aliased_grid(precision):
     P[:20] = x[:20]
     got = sp.probe_sums(g, P, **PHYS)
     assert np.array_equal(got, o.probe_sum(P, PHYS["m"], False), equal_nan=True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_particle_sum_lj_wall_bit_exact(dim, precision):
+    """SmoothedParticles.sum(sys, LJ_potential, p) for every particle
+    (src/core.jl:271-291; examples/collapse_symplectic.jl:144-162 with
+    E_wall = 10 |g| H): bit-exact against the oracle on a dam break, before
+    and after device steps.  The lattice puts the fluid one dr from the
+    walls, so dr_wall = 1.5 dr here (the example's 0.95 dr only acts once the
+    fluid presses into the walls)."""
+    from smoothedparticles_hip import states
+    st = states.dam_break(dim, 0.05 if dim == 2 else 0.1)
+    p = st.params()
+    g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, precision)
+    g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
+    lj = dict(m=st.m, dr_wall=1.5 * st.dr, E_wall=10 * 9.8 * 2.0, eps=1e-16)
+    sp.init_reference_density(g, **p)
+    for k in range(2):
+        if k:
+            sp.step(g, 30, scheme="wcsph_se_rho0", **p)
+            sp.create_cell_list(g)
+        o = O.OracleSystem(st.R, st.bbox_lo, st.bbox_hi)
+        o.load(g.get("x"), None, None, None, g.get("type"))
+        o.create_cell_list()
+        ref = o.particle_sum_lj(**lj)
+        got = sp.particle_sums_lj_wall(g, **lj)
+        assert np.array_equal(got, ref)
+        assert np.count_nonzero(got) > 10 and np.all(got <= 0.0)
+        assert np.all(got[st.type == 1.0] == 0.0) if k == 0 else True

@@ -168,3 +168,54 @@ def test_probe_sum_fluid_only_equals_fluid_subsystem():
     P = rng.random((30, 3))
     P[:, 2] = 0.0
     assert np.array_equal(o.probe_sum(P, 0.5, fluid_only=True), o2.probe_sum(P, 0.5))
+
+
+# ---- SmoothedParticles.sum(sys, func, p), the per-particle form ------------
+
+def lj_brute(x, typ, h, m, dr_wall, E_wall, eps):
+    """All pairs (q == p included) with the FP64 predicate, math.fsum."""
+    out = []
+    for p_ in range(len(x)):
+        terms = []
+        for q_ in range(len(x)):
+            d = x[p_] - x[q_]
+            r = math.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+            if r > h:
+                continue
+            if typ[q_] == 1.0 and typ[p_] == 0.0 and r < dr_wall:
+                s = dr_wall / (r + eps)
+                terms.append(m * E_wall * (0.5 * s ** 2 - 0.25 * s ** 4 - 0.25))
+        out.append(math.fsum(terms))
+    return np.array(out)
+
+
+def test_particle_sum_lj_brute_force_and_closed_form():
+    """The wall potential of examples/collapse_symplectic.jl:144-162 summed
+    per particle: against an all-pairs brute force (p == q included,
+    math.fsum), and a single fluid-wall pair in closed form; walls and
+    fluid-fluid pairs contribute nothing."""
+    rng = np.random.default_rng(21)
+    n = 900
+    x = rng.random((n, 3)) * [1.0, 1.0, 0.0]
+    typ = (rng.random(n) < 0.3).astype(np.float64)
+    h, m, dr_wall, E_wall, eps = 0.06, 0.4, 0.03, 137.2, 1e-16
+    o = O.OracleSystem(h, np.zeros(3), np.array([1.0, 1.0, 0.0]))
+    o.load(x, type=typ)
+    o.create_cell_list()
+    got = o.particle_sum_lj(m, dr_wall, E_wall, eps)
+    ref = lj_brute(x, typ, h, m, dr_wall, E_wall, eps)
+    assert np.allclose(got, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    assert np.all(got[typ == 1.0] == 0.0) and np.count_nonzero(got) > 20
+    # one fluid particle, one wall particle at r = 0.02
+    x2 = np.array([[0.5, 0.5, 0.0], [0.52, 0.5, 0.0]])
+    o2 = O.OracleSystem(h, np.zeros(3), np.array([1.0, 1.0, 0.0]))
+    o2.load(x2, type=np.array([0.0, 1.0]))
+    o2.create_cell_list()
+    s = dr_wall / (0.02 + eps)
+    v = o2.particle_sum_lj(m, dr_wall, E_wall, eps)
+    assert v[0] == pytest.approx(m * E_wall * (0.5 * s * s - 0.25 * s ** 4 - 0.25), rel=1e-14)
+    assert v[1] == 0.0
+    # at r >= dr_wall nothing, and the potential is zero at r = dr_wall - eps
+    o2.field("x")[1, 0] = 0.5 + dr_wall
+    o2.create_cell_list()
+    assert o2.particle_sum_lj(m, dr_wall, E_wall, eps)[0] == 0.0
