@@ -1,0 +1,558 @@
+// Pair-interaction kernels (shared device code of pair64.hip and pair32.hip):
+// the device form of _apply_binary! /
+// apply_binary! (src/core.jl:94-129) for the registered operators.
+//
+// Work decomposition.  A tile is up to T targets of one cell (T = 8..64, a
+// template parameter).  One 64-lane wave handles a tile: lane = (target t,
+// slice s), t = lane % T, s = lane / T, with S = 64 / T slices.
+//
+// Candidates.  The candidates of a target are the cells key + key_diff of
+// the reference; with di innermost, each of the 3 (2D) or 9 (3D) (dj, dk)
+// rows is the contiguous cell range [key + L1*dj + L1*L2*dk - 1, ... + 1],
+// clamped to [0, key_max) exactly as the reference's linear bounds test
+// (src/core.jl:98), so rows wrap and small grids visit aliased cells as
+// often as the reference does.  Since the particles are cell-sorted, each
+// row is one contiguous particle range (periodic systems: at most two, with
+// an image shift each).  All targets of a tile share the cell, hence the
+// ranges: the tile's candidate list is the concatenation of its ranges.
+//
+// Culling.  Only the candidates within reach of the tile -- squared
+// distance to the bounding box of its targets not above the predicate's
+// threshold -- can be accepted by any target, about half of the list in 3D.
+// The list is streamed in 64-candidate steps; each step appends its kept
+// candidates (ballot + lane rank) to a ring of 128 LDS slots, and every 64
+// kept candidates form a chunk.  The kept order is the list order, and the
+// box and the cull test are the same in every pass, so all passes see the
+// same chunks.  The testing pass also records each tile's kept candidates
+// (and their ranges, for the image shifts of periodic systems); a pass that
+// loads its masks reads that list instead of streaming and culling again.
+//
+// Each chunk is handled in two phases:
+//   1. mask: lane (t, s) tests the chunk slots s + S*j (j < T) against
+//      target t and sets bit j of its T-bit mask for slot s + S*j -- or,
+//      after a density pass on the same cell list, loads that pass's mask.
+//      Interleaving the slices keeps the accepted counts of the lanes
+//      alike (neighbouring slots are neighbouring particles);
+//   2. interact: the chunk's accepted pairs are cut into runs spread over
+//      all 64 lanes (balanced pairs, below), so the pair arithmetic runs
+//      once per accepted pair, not once per candidate.
+// At the end the S slices of a target are combined with wave shuffles.
+// The masks of target t for a chunk form one 64-bit word (the T-bit mask
+// of slice s in bits [s*T, s*T + T)), stored at
+// mask_base(tile) + chunk * T + t.
+//
+// Predicate (src/core.jl:104-105, src/algebra.jl:49-60): accept q != p when
+// RN(sqrt((d1*d1 + d2*d2) + d3*d3)) <= h.  sqrt is correctly rounded and
+// monotonic, so this equals s <= s_thr with s_thr = max{s : RN(sqrt(s)) <= h}
+// (computed on the host).  The FP64 kernels evaluate s without FMA
+// (-ffp-contract=off), i.e. bit-exactly as the reference.  The mixed
+// kernels evaluate |q|^2 - 2 p.q + |p|^2 in FP32 on tile-relative
+// coordinates (three FMAs per candidate): below s_thr (1 - BAND) the pair is
+// accepted, above s_thr (1 + BAND) rejected, and in between the exact FP64
+// test decides, so their neighbour sets (and masks) are the reference's
+// too.  The box test never drops a candidate that the predicate accepts:
+// in FP64 each box distance component is a correctly rounded difference no
+// larger than the target's, and the mixed test uses the same guard band.
+// The identity exclusion p != q is applied once per chunk to the slots that
+// hold p: the tile's own targets are always kept, so each occurrence of the
+// tile's cell in the list puts them in consecutive slots.
+#pragma once
+
+#include <type_traits>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+#define COL(c, f, i) ((c).cols[(long long)(f) * (c).cap + (i)])
+
+// Guard band of the mixed predicate, relative to s_thr.  Tile-relative
+// coordinates are below 4 h in magnitude; the FP32 expanded form
+// |q|^2 - 2 p.q + |p|^2 is then within ~1e-5 * h^2 of the exact squared
+// distance and the direct form within ~1e-6 * h^2; the band is 10x wider.
+static constexpr double BAND = 1.0e-4;
+
+__device__ inline void count_pairs(const PairCtx& c, unsigned n)
+{
+    // wave reduction of the per-lane counts, one atomic per wave
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o);
+    if (__lane_id() == 0 && n) atomicAdd(c.pair_counter, (unsigned long long)n);
+}
+
+// The wave's distance tests (the same value in every lane).
+__device__ inline void count_tests(const PairCtx& c, unsigned long long n)
+{
+    if (c.test_counter && __lane_id() == 0 && n) atomicAdd(c.test_counter, n);
+}
+
+// Compensated (2Sum) addition of a chunk partial into (s, e).
+__device__ inline void two_sum_add(float& s, float& e, float x)
+{
+    const float t = s + x;
+    const float bp = t - s;
+    e += (s - (t - bp)) + (x - bp);
+    s = t;
+}
+
+// Tile queues.  The tile range [0, ntiles) is cut into NQUEUE contiguous
+// parts, one per queue head.  A wave starts on queue blockIdx % NQUEUE (the
+// XCD it runs on, so neighbouring tiles share that XCD's L2), takes qgrab
+// tiles per atomic, and when its queue is empty moves on to the next ones.
+// With one tile per grab the tiles in flight on an XCD are consecutive and
+// share their neighbour cells in the L2 (3D: 9.6 instead of 10.6 ms per
+// dam-break step); the short 2D tiles take four per grab, which spares
+// atomics.
+// Every wave leaves once all queues are past their ends.  The state is
+// wave-uniform (readfirstlane), so the tile record, the system id and the
+// system's parameters are scalar loads held in SGPRs.
+struct TileCursor {
+    int q0, k, cur, end;
+};
+
+__device__ inline void tile_cursor_init(TileCursor& tc)
+{
+    tc.q0 = (int)(blockIdx.x % NQUEUE);
+    tc.k = 0;
+    tc.cur = tc.end = 0;
+}
+
+__device__ inline int next_tile(const PairCtx& c, int ntiles, TileCursor& tc)
+{
+    if (tc.cur < tc.end) return tc.cur++;
+    while (tc.k < NQUEUE) {
+        const int q = (tc.q0 + tc.k) % NQUEUE;
+        const int lo = (int)((long long)ntiles * q / NQUEUE);
+        const int hi = (int)((long long)ntiles * (q + 1) / NQUEUE);
+        int start = 0;
+        if (__lane_id() == 0) start = atomicAdd(&c.queue[q * QSTRIDE], c.qgrab);
+        start = __builtin_amdgcn_readfirstlane(start);
+        if (start < hi - lo) {
+            tc.cur = lo + start;
+            tc.end = min(hi, lo + start + c.qgrab);
+            return tc.cur++;
+        }
+        ++tc.k;
+    }
+    return ntiles;
+}
+
+__device__ inline long long mask_base(const int4& td)
+{
+    return (long long)(unsigned)td.z | ((long long)td.w << 32);
+}
+
+// m * 2 + (e <= thr): the predicate shifted into a mask, as one compare
+// into VCC and one add with carry (the compiler would spend a select, a
+// shift and an OR per bit).  NaN compares false, as e <= thr does.
+__device__ inline unsigned shift_in_le(unsigned m, float e, float thr)
+{
+    unsigned r;
+    asm("v_cmp_ge_f32_e32 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+        : "=v"(r)
+        : "v"(m), "v"(thr), "v"(e)
+        : "vcc");
+    return r;
+}
+
+__device__ inline unsigned shift_in_le(unsigned m, double e, double thr)
+{
+    unsigned r;
+    asm("v_cmp_ge_f64_e32 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+        : "=v"(r)
+        : "v"(m), "v"(thr), "v"(e)
+        : "vcc");
+    return r;
+}
+
+// Per-lane mask of T slots.
+template <int T>
+using MaskT = typename std::conditional<(T > 32), unsigned long long, unsigned>::type;
+
+template <int T>
+__device__ inline int lowest_bit(MaskT<T> m)
+{
+    if constexpr (T > 32) return __ffsll((long long)m) - 1;
+    else return __ffs(m) - 1;
+}
+
+// ------------------------------------------------------------ candidates --
+// The candidate ranges of a tile: its cell's stencil rows, concatenated.
+// pref[i] candidates precede range i; a chunk's slot l holds candidate
+// base + l, whose range i satisfies pref[i] <= base + l < pref[i + 1].
+// The kept candidates are staged in a ring of 128 slots: q (and rg, for
+// periodic systems) of each slot, and the kept position of the first
+// target at each occurrence of the tile's cell.
+template <int ROWS, bool PER>
+struct Cands {
+    static constexpr int MAXR = 2 * ROWS;
+    static constexpr int NSH = PER ? MAXR : 1;
+    int j0[MAXR];
+    int pref[MAXR + 1];
+    double sx[NSH], sy[NSH], sz[NSH];  // image shift (periodic systems)
+    int q[128];                        // particle index of each ring slot
+    int rg[PER ? 128 : 1];             // range of each ring slot
+    int self0[MAXR];
+};
+
+// Uniform work (every lane computes the same values); lane 0 writes.
+// Returns the number of ranges; *total receives the candidate count.
+template <int ROWS, bool PERIODIC>
+__device__ inline int build_cands(const SysDesc& d, const int* cell_start, int lc,
+                                  Cands<ROWS, PERIODIC>& L, int& total)
+{
+    const bool w = __lane_id() == 0;
+    int n = 0, tot = 0;
+    for (int row = 0; row < ROWS; ++row) {
+        Seg segs[2];
+        const int ns = row_segments<ROWS, PERIODIC>(d, cell_start, lc, row, segs);
+        for (int k = 0; k < 2; ++k) {
+            if (k < ns) {
+                if (w) {
+                    L.j0[n] = segs[k].j0;
+                    L.pref[n] = tot;
+                    if constexpr (PERIODIC) {
+                        L.sx[n] = segs[k].sx;
+                        L.sy[n] = segs[k].sy;
+                        L.sz[n] = segs[k].sz;
+                    }
+                }
+                tot += segs[k].j1 - segs[k].j0;
+                ++n;
+            }
+        }
+    }
+    if (w) L.pref[n] = tot;
+    total = tot;
+    return n;
+}
+
+// Slot of chunk `base` that holds candidate g (g < total): its range and
+// particle index.  Ranges are scanned from `cur`, the range of slot 0.
+template <int ROWS, bool PER>
+__device__ inline int slot_range(const Cands<ROWS, PER>& L, int cur, int g)
+{
+    int i = cur;
+    while (L.pref[i + 1] <= g) ++i;
+    return i;
+}
+
+// Range of the next chunk's slot 0.
+template <int ROWS, bool PER>
+__device__ inline int advance_range(const Cands<ROWS, PER>& L, int nr, int cur, int next_base)
+{
+    while (cur < nr && L.pref[cur + 1] <= next_base) ++cur;
+    return cur;
+}
+
+template <int ROWS>
+__device__ inline Seg range_shift(const Cands<ROWS, true>& L, int i)
+{
+    Seg g;
+    g.j0 = g.j1 = 0;
+    g.sx = L.sx[i];
+    g.sy = L.sy[i];
+    g.sz = L.sz[i];
+    return g;
+}
+
+// Lanes below this one with their bit set in m.
+__device__ inline int lane_rank(unsigned long long m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Append the lanes' kept candidates to the ring (list order).  fill: kept so
+// far; nself: occurrences of the first target so far.  Returns the lane's
+// kept position (its ring slot is pos & 127), -1 if its candidate is not
+// kept.
+template <int ROWS, bool PER>
+__device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool keep, int q, int rg,
+                              int first)
+{
+    const unsigned long long km = __ballot(keep);
+    const unsigned long long sm = __ballot(keep && q == first);
+    const int pos = fill + lane_rank(km);
+    if (keep) {
+        const int r = pos & 127;
+        L.q[r] = q;
+        if constexpr (PER) L.rg[r] = rg;
+        if (q == first) L.self0[nself + lane_rank(sm)] = pos;
+    }
+    fill += __popcll(km);
+    nself += __popcll(sm);
+    return keep ? pos : -1;
+}
+
+// Clear the bits of the chunk's slots (kept positions head ... head + 63)
+// that hold the target itself (the reference skips p === q,
+// src/core.jl:105): target t sits t slots after the first target at every
+// occurrence of the tile's cell.
+template <int ROWS, bool PER, int T>
+__device__ inline void drop_self(const Cands<ROWS, PER>& L, int nself, int head, int t, int s,
+                                 MaskT<T>& m)
+{
+    constexpr int S = 64 / T;
+    for (int k = 0; k < nself; ++k) {
+        const int l = L.self0[k] + t - head;
+        if (l >= 0 && l < 64 && l % S == s) m &= ~(MaskT<T>(1) << (l / S));
+    }
+}
+
+// Bounding box of a tile's targets, computed with the tile table in absolute
+// FP64 coordinates; the mixed kernels convert it to FP32 relative to the
+// tile's first target.
+template <class V>
+struct Box {
+    V x0, x1, y0, y1, z0, z1;
+};
+
+// Squared distance of (x, y, z) to the box (0 inside), evaluated in the
+// predicate's operation order.
+template <bool FLAT>
+__device__ inline double box_dist2(const Box<double>& b, double x, double y, double z)
+{
+    const double dx = fmax(fmax(b.x0 - x, x - b.x1), 0.0);
+    const double dy = fmax(fmax(b.y0 - y, y - b.y1), 0.0);
+    double ss = dx * dx + dy * dy;
+    if (!FLAT) {
+        const double dz = fmax(fmax(b.z0 - z, z - b.z1), 0.0);
+        ss = ss + dz * dz;
+    }
+    return ss;
+}
+
+template <bool FLAT>
+__device__ inline float box_dist2(const Box<float>& b, float x, float y, float z)
+{
+    const float dx = fmaxf(fmaxf(b.x0 - x, x - b.x1), 0.0f);
+    const float dy = fmaxf(fmaxf(b.y0 - y, y - b.y1), 0.0f);
+    const float dz = FLAT ? 0.0f : fmaxf(fmaxf(b.z0 - z, z - b.z1), 0.0f);
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// ------------------------------------------------------ balanced pairs --
+// The accepted pairs of a chunk are spread over all 64 lanes instead of
+// being walked by the lane that owns the target: with ~15% acceptance the
+// per-lane counts of a chunk differ a lot (targets near the chunk's
+// particles accept many, far ones none), and an owner-walked loop runs as
+// long as the busiest lane.  Per chunk:
+//   - the owner lane of target t (slice 0, lane t) publishes the target's
+//     64-bit word of accepted slots in LDS;
+//   - the pairs are cut into runs of at most B pairs, one target per run,
+//     with B the smallest quota (from the mean up) for which the runs fit
+//     in 64 lanes; lane l takes run l;
+//   - a lane reads its target's data from the owner lane (shuffles), skips
+//     to its run with a popcount descent and walks the run;
+//   - its partial sums go to LDS, and the owner folds the runs of its target
+//     in run order, so the result does not depend on timing.
+// The words are exact (the mixed test settles its guard band before), so
+// the runs -- and the summation order -- are the same whether a pass tests
+// the candidates or loads the density pass's words.
+template <class Part>
+struct Bal {
+    unsigned long long w[64];  // accepted slots of each target
+    int lane_t[64];            // target of each lane's run (-1: none)
+    Part part[64];             // each lane's partial sums of the chunk
+};
+
+// Bit b of a target's word is slot (b % T) * (64 / T) + b / T of the chunk.
+template <int T>
+__device__ inline int slot_of_bit(int b)
+{
+    return (b % T) * (64 / T) + b / T;
+}
+
+// w without its n lowest set bits (0 <= n < popcount(w)).
+__device__ inline unsigned long long drop_low_bits(unsigned long long w, int n)
+{
+    if (n <= 0) return w;
+    unsigned long long x = w;
+    int pos = 0;
+    int c = __popc((unsigned)x);
+    if (n >= c) { n -= c; x >>= 32; pos += 32; }
+    c = __popc((unsigned)x & 0xffffu);
+    if (n >= c) { n -= c; x >>= 16; pos += 16; }
+    c = __popc((unsigned)x & 0xffu);
+    if (n >= c) { n -= c; x >>= 8; pos += 8; }
+    c = __popc((unsigned)x & 0xfu);
+    if (n >= c) { n -= c; x >>= 4; pos += 4; }
+    c = __popc((unsigned)x & 0x3u);
+    if (n >= c) { n -= c; x >>= 2; pos += 2; }
+    c = (int)((unsigned)x & 1u);
+    if (n >= c) pos += 1;
+    return w & (~0ull << pos);
+}
+
+__device__ inline int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Copy of v as held by lane src.
+template <class V>
+__device__ inline V shfl_value(const V& v, int src)
+{
+    static_assert(sizeof(V) % 4 == 0, "shuffled per 32-bit word");
+    V r;
+    const int* a = reinterpret_cast<const int*>(&v);
+    int* b = reinterpret_cast<int*>(&r);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(V) / 4; ++i) b[i] = __shfl(a[i], src);
+    return r;
+}
+
+// Wave sum of v, 0 <= v < 128, from bit-sliced ballots: no cross-lane
+// data movement through LDS, the result is uniform.
+__device__ inline int wave_sum7(int v)
+{
+    int sum = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) sum += __popcll(__ballot((v >> b) & 1)) << b;
+    return sum;
+}
+
+// Sum of v over the lanes below this one, 0 <= v < 128.
+__device__ inline int lanes_below_sum7(int v)
+{
+    int sum = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) sum += lane_rank(__ballot((v >> b) & 1)) << b;
+    return sum;
+}
+
+// ceil(65536 / d) for a uniform 1 <= d <= 64: a reciprocal estimate, then
+// corrected to the exact value.
+__device__ inline int run_magic(int d)
+{
+    int m = (int)(65536.0f * __builtin_amdgcn_rcpf((float)d)) + 1;
+    while ((m - 1) * d >= 65536) --m;
+    while (m * d < 65536) ++m;
+    return m;
+}
+
+// ceil(pop / B) for 0 <= pop <= 64: with m = ceil(65536 / B) and B <= 64,
+// floor(n m / 65536) = floor(n / B) for n < 1024.
+__device__ inline int runs_of(int pop, int B, int m)
+{
+    if (pop <= 0) return 0;
+    return B <= 64 ? ((pop + B - 1) * m) >> 16 : (pop + B - 1) / B;
+}
+
+// Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
+// only, 0 elsewhere).  Returns the total; sets the lane's run (target tt,
+// first rank r0, count cnt) and, for owner lanes, the first run and the
+// number of runs of their target.
+template <class Part>
+__device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int& r0, int& cnt,
+                                 int& first_run, int& nruns)
+{
+    const int tot = wave_sum7(pop);
+    tt = -1;
+    r0 = cnt = first_run = nruns = 0;
+    if (tot == 0) return 0;
+    int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
+    int m = run_magic(B);
+    int need = wave_sum7(runs_of(pop, B, m));
+    if (need > 64) {
+        ++B;
+        m = B <= 64 ? run_magic(B) : 0;
+        need = wave_sum7(runs_of(pop, B, m));
+        if (need > 64) {
+            const int nz = __popcll(__ballot(pop > 0));
+            B = nz < 64 ? (tot + (64 - nz) - 1) / (64 - nz) : wave_max(pop);
+            m = B <= 64 ? run_magic(B) : 0;
+        }
+    }
+    nruns = runs_of(pop, B, m);
+    first_run = lanes_below_sum7(nruns);
+    bal.lane_t[lane] = -1;
+    __syncthreads();
+    for (int k = 0; k < nruns; ++k) bal.lane_t[first_run + k] = lane;
+    __syncthreads();
+    tt = bal.lane_t[lane];
+    const int src = tt < 0 ? 0 : tt;
+    const int st = __shfl(first_run, src);
+    const int pt = __shfl(pop, src);
+    if (tt >= 0) {
+        r0 = (lane - st) * B;
+        cnt = min(B, pt - r0);
+    }
+    return tot;
+}
+
+// ------------------------------------------------------------- staging --
+// A candidate's fields as loaded from the columns (the image shift already
+// added to the position).  The kernels load the next chunk's candidates into
+// registers before working on the current chunk and write them to LDS at
+// the next iteration, so the loads' latency overlaps the tests and pairs.
+// Each operator fills (fetch) and writes (put) only the fields it uses.
+struct Raw {
+    double x, y, z, a0, a1, a2, a3, a4;
+};
+
+// Slots past the end of the list: far away, rejected by every test.
+__device__ inline Raw pad_raw()
+{
+    Raw r{};
+    r.x = r.y = r.z = 1.0e30;
+    return r;
+}
+
+template <bool P>
+__device__ inline void fetch_pos(const PairCtx& c, int q, const Seg& g, Raw& r)
+{
+    r.x = shifted<P>(COL(c, C_X, q), g.sx);
+    r.y = shifted<P>(COL(c, C_Y, q), g.sy);
+    r.z = shifted<P>(COL(c, C_Z, q), g.sz);
+}
+
+// Fetch candidate g of the list (a pad past its end) into registers; cur is
+// the range of the chunk's first slot.  rq, rg: its particle index and range.
+template <int ROWS, bool PERIODIC, class Op>
+__device__ inline void fetch_slot(const PairCtx& c, const Cands<ROWS, PERIODIC>& L, int total,
+                                  int cur, int g, Raw& raw, int& rq, int& rg)
+{
+    if (g < total) {
+        const int i = slot_range(L, cur, g);
+        const int q = L.j0[i] + (g - L.pref[i]);
+        Seg sh{};
+        if constexpr (PERIODIC) sh = range_shift(L, i);
+        raw = Op::template fetch<PERIODIC>(c, q, sh);
+        rq = q;
+        rg = i;
+    } else {
+        raw = pad_raw();
+        rq = 0;
+        rg = 0;
+    }
+}
+
+// Neighbour-set digest of each target: the number of accepted q and the sum
+// modulo 2^64 of mix64(refidx q + 1), written to pair_out[2 refidx p] and
+// pair_out[2 refidx p + 1].  Integer sums, so the result does not depend on
+// the order in which the pairs are met.
+__device__ inline unsigned long long mix64(unsigned long long z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+__device__ inline void digest_write(const PairCtx& c, int p, unsigned long long h,
+                                    unsigned long long n, int T, bool write)
+{
+    for (int o = T; o < 64; o <<= 1) {
+        h += __shfl_xor(h, o);
+        n += __shfl_xor(n, o);
+    }
+    if (!write) return;
+    const long long r = c.refidx[p];
+    c.pair_out[2 * r] = (long long)n;
+    c.pair_out[2 * r + 1] = (long long)h;
+}
+
+}  // namespace sph
