@@ -215,6 +215,21 @@ int sph_particle_sum(sph_system* s, int func, const double* fparams, int nfparam
 int sph_probe_sum_noalloc(sph_system* s, const sph_params* params, int flags, const double* x,
                           int64_t np, double* out);
 
+/* save_frame! (src/IO.jl:53-75) without stopping the steps: the fields
+ * are gathered into reference order on the system's stream (so they are the
+ * state at the time of the call) and copied to pinned host memory on a
+ * second stream while later steps run.  sph_snapshot_wait blocks until the
+ * copy is done; sph_snapshot_field then gives field k's n or 3n doubles
+ * ([n][3] for vector fields), valid until sph_snapshot_free.  Up to
+ * SPH_SNAPSHOT_MAX_FIELDS fields; free every snapshot before destroying its
+ * handle. */
+#define SPH_SNAPSHOT_MAX_FIELDS 9
+typedef struct sph_snapshot sph_snapshot;
+int sph_snapshot_begin(sph_system* s, const int* fields, int nfields, sph_snapshot** out);
+int sph_snapshot_wait(sph_snapshot* snap, int64_t* n);
+int sph_snapshot_field(sph_snapshot* snap, int field, const double** data);
+int sph_snapshot_free(sph_snapshot* snap);
+
 /* Neighbour pairs of the current cell list as reference particle indices
  * (p, q), p != q, dist <= h: pairs[2k], pairs[2k+1].  The order is
  * unspecified.  *n receives the total; at most cap pairs are written. */

@@ -63,6 +63,9 @@ int sph_batch_build_cells(sph_batch* b);
 /* Neighbour pairs of one system under the current cell lists, as
  * sph_neighbour_pairs. */
 int sph_batch_neighbour_pairs(sph_batch* b, int id, int64_t* pairs, int64_t cap, int64_t* n);
+/* Asynchronous frame download of one system, as sph_snapshot_begin. */
+int sph_batch_snapshot_begin(sph_batch* b, int id, const int* fields, int nfields,
+                             sph_snapshot** out);
 /* Neighbour-set digest of one system under the current cell lists, as
  * sph_neighbour_digest (n must equal the system's particle count). */
 int sph_batch_neighbour_digest(sph_batch* b, int id, int64_t* counts, uint64_t* hashes,

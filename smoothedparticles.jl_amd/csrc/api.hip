@@ -14,6 +14,7 @@
 
 #include "sph_batch.h"
 #include "sph_internal.h"
+#include "sph_setup.h"
 
 using namespace sph;
 
@@ -37,6 +38,13 @@ void enqueue_field_in(sph_system* s, long long off, long long n, int col0, int n
 void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const double* in_dev,
                     int has_v, int has_rho, int has_P, int has_type);
 void enqueue_set_int(sph_system* s, int* p, int v);
+void enqueue_set_meta(sph_system* s, int idx, int v);
+long long lattice_plan(const sph_lattice_spec* spec, LatticeGeo* g, std::vector<double>* ball);
+void enqueue_lattice_flags(sph_system* s, const LatticeGeo& g, const double* ball, long long nlat,
+                           unsigned* flags, unsigned* pos, unsigned* part);
+void enqueue_lattice_write(sph_system* s, const LatticeGeo& g, const sph_lattice_spec* spec,
+                           const unsigned* flags, const unsigned* pos, long long nlat, int slot,
+                           int ref0);
 void enqueue_fill_int(sph_system* s, int* p, int v, long long n);
 void enqueue_remap_sys(sph_system* s, const int* map_dev);
 long long scan_partials_needed(long long n);
@@ -586,6 +594,10 @@ int destroy(sph_system* s)
         (void)hipEventDestroy(e.second.first);
         (void)hipEventDestroy(e.second.second);
     }
+    if (s->copy_stream) {
+        (void)hipStreamSynchronize(s->copy_stream);
+        (void)hipStreamDestroy(s->copy_stream);
+    }
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return 0;
@@ -667,6 +679,64 @@ int append_particles(sph_system* s, int slot, int64_t n, const double* x, const 
     s->n += n;
     s->cells_valid = s->masks_valid = false;
     return 0;
+}
+
+// Device-side generate_particles! of a lattice (sph_setup.h) into system
+// `slot`: plan on the host, flag and scan on the device, one sync for the
+// count, then the particles are written where sph_append would put them.
+int generate_lattice(sph_system* s, int slot, const sph_lattice_spec* spec, int64_t* n_added)
+{
+    if (n_added) *n_added = 0;
+    if (s->sys_n[slot] > 0 && s->sys_off[slot] + s->sys_n[slot] != s->n)
+        return fail(SPH_EINVAL, "particles can only be appended to the last system of a batch");
+    LatticeGeo g;
+    std::vector<double> ball;
+    const long long nlat = lattice_plan(spec, &g, &ball);
+    if (nlat == -1) return fail(SPH_EINVAL, "invalid lattice specification");
+    if (nlat < 0) return fail(SPH_EINVAL, "lattice covering above 2^31 points");
+    if (nlat == 0) return 0;
+    unsigned *flags = nullptr, *pos = nullptr, *part = nullptr;
+    double* dball = nullptr;
+    const long long nparts = scan_partials_needed(nlat);
+    int rc = dalloc(s, &flags, nlat);
+    if (!rc) rc = dalloc(s, &pos, nlat + 1);
+    if (!rc) rc = dalloc(s, &part, nparts);
+    if (!rc && !ball.empty()) rc = dalloc(s, &dball, (long long)ball.size());
+    unsigned total = 0;
+    hipError_t e = hipSuccess;
+    if (!rc && dball)
+        e = hipMemcpyAsync(dball, ball.data(), sizeof(double) * ball.size(), hipMemcpyHostToDevice,
+                           s->stream);
+    if (!rc && e == hipSuccess) {
+        enqueue_lattice_flags(s, g, dball, nlat, flags, pos, part);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&total, pos + nlat, sizeof(unsigned), hipMemcpyDeviceToHost,
+                               s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
+    if (!rc && e != hipSuccess) rc = fail(SPH_EHIP, std::string("lattice: ") + hipGetErrorString(e));
+    if (!rc && (long long)total > (1ll << 31) - 64 - s->n)
+        rc = fail(SPH_EINVAL, "particle count exceeds 2^31");
+    if (!rc && total > 0) rc = ensure_capacity(s, s->n + total);
+    if (!rc && total > 0) {
+        enqueue_lattice_write(s, g, spec, flags, pos, nlat, slot, (int)s->sys_n[slot]);
+        enqueue_set_meta(s, META_N, (int)(s->n + total));
+        if (s->sys_n[slot] == 0) s->sys_off[slot] = s->n;
+        s->sys_n[slot] += total;
+        enqueue_set_int(s, s->sys_cnt + SYSCNT * slot + SC_N, (int)s->sys_n[slot]);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = fail(SPH_EHIP, std::string("lattice: ") + hipGetErrorString(e));
+        s->n += total;
+        s->cells_valid = s->masks_valid = false;
+        if (!rc && n_added) *n_added = total;
+    }
+    dfree(s, flags, nlat);
+    dfree(s, pos, nlat + 1);
+    dfree(s, part, nparts);
+    dfree(s, dball, (long long)ball.size());
+    return rc;
 }
 
 int build_cells(sph_system* s)
@@ -1004,6 +1074,87 @@ int probe_sum(sph_system* s, int slot, double m, int flags, const double* x, lon
     const hipError_t ef = hipFree(buf);
     if (e != hipSuccess) return fail(SPH_EHIP, std::string("probe sum: ") + hipGetErrorString(e));
     if (ef != hipSuccess) return fail(SPH_EHIP, std::string("hipFree: ") + hipGetErrorString(ef));
+    return 0;
+}
+
+}  // namespace
+
+// An asynchronous frame download (sph_snapshot_begin): the fields are
+// gathered into reference order on the system's stream, then copied to
+// pinned host memory on a second stream while the system's stream goes on
+// with the next steps.
+struct sph_snapshot {
+    sph_system* s;
+    long long n;
+    int nf;
+    int field[SPH_SNAPSHOT_MAX_FIELDS];
+    long long off[SPH_SNAPSHOT_MAX_FIELDS];
+    double* dev;
+    double* host;
+    long long total;
+    hipEvent_t gathered, done;
+};
+
+namespace {
+
+int snapshot_free(sph_snapshot* q)
+{
+    if (!q) return 0;
+    if (q->done) (void)hipEventSynchronize(q->done);
+    if (q->dev) (void)hipFree(q->dev);
+    if (q->host) (void)hipHostFree(q->host);
+    if (q->gathered) (void)hipEventDestroy(q->gathered);
+    if (q->done) (void)hipEventDestroy(q->done);
+    delete q;
+    return 0;
+}
+
+int snapshot_begin(sph_system* s, int slot, const int* fields, int nf, sph_snapshot** out)
+{
+    *out = nullptr;
+    if (nf < 1 || nf > SPH_SNAPSHOT_MAX_FIELDS) return fail(SPH_EINVAL, "1 to 9 fields per snapshot");
+    sph_snapshot* q = new sph_snapshot();
+    q->s = s;
+    q->n = s->sys_n[slot];
+    q->nf = nf;
+    long long total = 0;
+    for (int k = 0; k < nf; ++k) {
+        int nc;
+        if (fields[k] == SPH_FIELD_KEY || col_of_field(fields[k], &nc) < 0) {
+            delete q;
+            return fail(SPH_EINVAL, "unknown snapshot field");
+        }
+        q->field[k] = fields[k];
+        q->off[k] = total;
+        total += nc * q->n;
+    }
+    q->total = total;
+    hipError_t e = hipSuccess;
+    if (!s->copy_stream) e = hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&q->gathered, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&q->done, hipEventDisableTiming);
+    if (e == hipSuccess && total > 0) e = hipMalloc((void**)&q->dev, sizeof(double) * total);
+    if (e == hipSuccess && total > 0) e = hipHostMalloc((void**)&q->host, sizeof(double) * total);
+    if (e == hipSuccess) {
+        for (int k = 0; k < nf; ++k) {
+            int nc;
+            const int col = col_of_field(q->field[k], &nc);
+            enqueue_field_out(s, s->sys_off[slot], q->n, col, nc, q->dev + q->off[k]);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(q->gathered, s->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s->copy_stream, q->gathered, 0);
+    if (e == hipSuccess && total > 0)
+        e = hipMemcpyAsync(q->host, q->dev, sizeof(double) * total, hipMemcpyDeviceToHost,
+                           s->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(q->done, s->copy_stream);
+    if (e != hipSuccess) {
+        const std::string msg = std::string("snapshot: ") + hipGetErrorString(e);
+        snapshot_free(q);
+        return fail(SPH_EHIP, msg);
+    }
+    *out = q;
     return 0;
 }
 
@@ -1430,6 +1581,60 @@ int sph_probe_sum_noalloc(sph_system* s, const sph_params* params, int flags, co
     if (!s || !params || (np > 0 && (!x || !out))) return fail(SPH_EINVAL, "null argument");
     if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
     return probe_sum_noalloc(s, params->m, flags, x, np, out);
+}
+
+int sph_snapshot_begin(sph_system* s, const int* fields, int nfields, sph_snapshot** out)
+{
+    if (!s || !fields || !out) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return snapshot_begin(s, 0, fields, nfields, out);
+}
+
+int sph_batch_snapshot_begin(sph_batch* b, int id, const int* fields, int nfields,
+                             sph_snapshot** out)
+{
+    if (!b || !fields || !out) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return snapshot_begin(b, slot, fields, nfields, out);
+}
+
+int sph_snapshot_wait(sph_snapshot* q, int64_t* n)
+{
+    if (!q) return fail(SPH_EINVAL, "null snapshot");
+    HIPC(hipEventSynchronize(q->done));
+    if (n) *n = q->n;
+    return 0;
+}
+
+int sph_snapshot_field(sph_snapshot* q, int field, const double** data)
+{
+    if (!q || !data) return fail(SPH_EINVAL, "null argument");
+    HIPC(hipEventSynchronize(q->done));
+    for (int k = 0; k < q->nf; ++k)
+        if (q->field[k] == field) {
+            *data = q->host ? q->host + q->off[k] : nullptr;
+            return 0;
+        }
+    return fail(SPH_EINVAL, "field not in the snapshot");
+}
+
+int sph_snapshot_free(sph_snapshot* q) { return snapshot_free(q); }
+
+int sph_generate_lattice(sph_system* s, const sph_lattice_spec* spec, int64_t* n_added)
+{
+    if (!s || !spec) return fail(SPH_EINVAL, "null argument");
+    if (nsys(s) == 0) return fail(SPH_ESTATE, "the handle holds no system");
+    return generate_lattice(s, 0, spec, n_added);
+}
+
+int sph_batch_generate_lattice(sph_batch* b, int id, const sph_lattice_spec* spec,
+                               int64_t* n_added)
+{
+    if (!b || !spec) return fail(SPH_EINVAL, "null argument");
+    const int slot = slot_of(b, id);
+    if (slot < 0) return fail(SPH_EINVAL, "unknown system id");
+    return generate_lattice(b, slot, spec, n_added);
 }
 
 int sph_particle_sum(sph_system* s, int func, const double* fparams, int nfparams, double* out,

@@ -80,6 +80,25 @@ std::vector<double> ball_offsets(int dim, double dr, double w)
 
 }  // namespace
 
+namespace sph {
+// For the device generator (generate.hip): the lattice index range of the
+// covering of a box, and the offsets of covering(grid, Ball(0, 0, 0, w)).
+void lattice_range(int dim, double dr, const double* lo, const double* hi, int64_t* rlo,
+                   int64_t* rhi)
+{
+    Box3 b{{lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+    const Range3 r = covering_range(dim, dr, b);
+    for (int d = 0; d < 3; ++d) {
+        rlo[d] = r.lo[d];
+        rhi[d] = r.hi[d];
+    }
+}
+std::vector<double> lattice_ball_offsets(int dim, double dr, double w)
+{
+    return ball_offsets(dim, dr, w);
+}
+}  // namespace sph
+
 extern "C" int64_t sph_lattice_box(int dim, double dr, const double* bmin, const double* bmax,
                                    double* out, int64_t cap)
 {

@@ -145,6 +145,14 @@ enum PairOp {
     NPAIROP
 };
 
+// A lattice covering for the device generator (generate.hip).
+struct LatticeGeo {
+    int dim, kind, lid_axis, nball;
+    double dr, lid;
+    double lo[3], hi[3];          // the box (SPH_LATTICE_BOX) or the tank
+    long long r0[3], ext[3];      // covering index range: first index and extent per axis
+};
+
 // Unary stages of the collapse_dry time loop (k_cd_unary).
 enum CdStage { CD_KICK_MOVE = 0, CD_MOVE = 1, CD_KICK = 2 };
 
@@ -272,6 +280,7 @@ struct sph_system {
     long long cap;
     bool cells_valid;
     hipStream_t stream;
+    hipStream_t copy_stream;  // asynchronous frame downloads (created on first use)
     int device;          // the device the handle's memory and stream live on
 
     // systems

@@ -651,6 +651,8 @@ void enqueue_append(sph_system* s, int n0, int m, int ref0, int sys, const doubl
 
 void enqueue_set_int(sph_system* s, int* p, int v) { k_set_int<<<1, 1, 0, s->stream>>>(p, v); }
 
+void enqueue_set_meta(sph_system* s, int idx, int v) { k_set_meta<<<1, 1, 0, s->stream>>>(s->meta, idx, v); }
+
 void enqueue_fill_int(sph_system* s, int* p, int v, long long n)
 {
     if (n > 0) k_fill_int<<<nblk(n), 256, 0, s->stream>>>(p, v, n);

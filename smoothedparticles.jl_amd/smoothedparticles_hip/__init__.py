@@ -40,7 +40,7 @@ __all__ = [
     "wendland2", "rDwendland2", "wendland3", "rDwendland3", "FLUID", "WALL", "Batch",
     "probe_sums", "assemble_vector", "assemble_matrix", "drift", "kick", "PROGRAMS",
     "program_operators", "host_apply", "Particle", "neighbour_digest", "cell_keys",
-    "visit_order", "particle_sums_lj_wall", "collapse_dry_init",
+    "visit_order", "particle_sums_lj_wall", "collapse_dry_init", "Snapshot",
 ]
 
 FLUID = 0.0
@@ -235,6 +235,17 @@ class ParticleSystem:
                                _as_ptr(P) if P is not None else None,
                                _as_ptr(t) if t is not None else None))
 
+    def snapshot(self, *names):
+        """Start an asynchronous download of the named fields (Snapshot)."""
+        return Snapshot(lambda ids, k, out: lib().sph_snapshot_begin(self._h, ids, k, out), names)
+
+    def generate_lattice(self, spec):
+        """Device-side generate_particles! of a lattice (include/sph_setup.h,
+        sph_generate_lattice); returns the number of particles added."""
+        n = ctypes.c_int64()
+        check(lib().sph_generate_lattice(self._h, ctypes.byref(spec), ctypes.byref(n)))
+        return n.value
+
     def get(self, name):
         n = len(self)
         shape = (n, 3) if name in N.VECTOR_FIELDS else (n,)
@@ -311,6 +322,52 @@ class ParticleSystem:
         n = ctypes.c_int64()
         check(lib().sph_profile_read(self._h, kernel, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class Snapshot:
+    """An asynchronous download of fields (sph_snapshot_begin): started on
+    the device at creation, in the order of the steps; wait() returns the
+    arrays (reference order) once the copy is done.  Close it (or use it as
+    a context manager) before the system is destroyed."""
+
+    def __init__(self, begin, names):
+        self.names = list(names)
+        for nm in self.names:
+            if nm not in N.FIELDS:
+                raise KeyError(f"Variable {nm} does not exist!")
+        ids = (ctypes.c_int * len(self.names))(*[N.FIELDS[nm] for nm in self.names])
+        self._q = ctypes.c_void_p()
+        check(begin(ids, len(self.names), ctypes.byref(self._q)))
+
+    def wait(self):
+        n = ctypes.c_int64()
+        check(lib().sph_snapshot_wait(self._q, ctypes.byref(n)))
+        out = {}
+        for nm in self.names:
+            ptr = _dptr()
+            check(lib().sph_snapshot_field(self._q, N.FIELDS[nm], ctypes.byref(ptr)))
+            k = 3 if nm in N.VECTOR_FIELDS else 1
+            a = (np.ctypeslib.as_array(ptr, shape=(n.value * k,)).copy() if n.value
+                 else np.empty(0))
+            out[nm] = a.reshape(n.value, 3) if k == 3 else a
+        return out
+
+    def close(self):
+        if self._q:
+            lib().sph_snapshot_free(self._q)
+            self._q = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ParticleField:
@@ -743,6 +800,19 @@ class Batch:
                                   _as_ptr(P) if P is not None else None,
                                   _as_ptr(t) if t is not None else None, ctypes.byref(sid)))
         return sid.value
+
+    def snapshot(self, sid, *names):
+        """Start an asynchronous download of fields of system sid (Snapshot)."""
+        return Snapshot(lambda ids, k, out: lib().sph_batch_snapshot_begin(self._h, int(sid), ids,
+                                                                           k, out), names)
+
+    def generate_lattice(self, sid, spec):
+        """Device-side generate_particles! of a lattice into system sid (the
+        last system holding particles); returns the number added."""
+        n = ctypes.c_int64()
+        check(lib().sph_batch_generate_lattice(self._h, int(sid), ctypes.byref(spec),
+                                               ctypes.byref(n)))
+        return n.value
 
     def add_state(self, st, periodic=False):
         """Add a workload state of ``states`` (dam break, Taylor-Green)."""

@@ -25,6 +25,7 @@ SPH_MIXED = 1
 FIELDS = {"x": 0, "v": 1, "a": 2, "rho": 3, "P": 4, "Drho": 5, "rho0": 6, "type": 7}
 VECTOR_FIELDS = {"x", "v", "a"}
 SPH_FIELD_KEY = 8  # read only: find_key at the last build (1-based)
+SPH_SNAPSHOT_MAX_FIELDS = 9
 
 # operator ids
 SPH_OP_DENSITY_SUM = 1
@@ -90,6 +91,32 @@ class sph_system_desc(ctypes.Structure):
     ]
 
 
+class sph_lattice_spec(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("kind", ctypes.c_int),
+        ("dr", ctypes.c_double),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("width", ctypes.c_double),
+        ("lid_axis", ctypes.c_int),
+        ("hydro_axis", ctypes.c_int),
+        ("lid", ctypes.c_double),
+        ("type", ctypes.c_double),
+        ("rho", ctypes.c_double),
+        ("P", ctypes.c_double),
+        ("rho0", ctypes.c_double),
+        ("g", ctypes.c_double),
+        ("H", ctypes.c_double),
+        ("c", ctypes.c_double),
+        ("jitter", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+SPH_LATTICE_BOX = 0
+SPH_LATTICE_WALL_LAYER = 1
+
 _d = ctypes.c_double
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -148,6 +175,13 @@ SIGNATURES = {
     "sph_lattice_box": (_i64, [_i, _d, _pd, _pd, _pd, _i64]),
     "sph_lattice_wall_layer": (_i64, [_i, _d, _pd, _pd, _d, _i, _d, _pd, _i64]),
     "sph_jitter": (None, [_pd, _i64, _i, _d, ctypes.c_uint64]),
+    "sph_generate_lattice": (_i, [_p, ctypes.POINTER(sph_lattice_spec), _pi64]),
+    "sph_snapshot_begin": (_i, [_p, ctypes.POINTER(_i), _i, ctypes.POINTER(_p)]),
+    "sph_batch_snapshot_begin": (_i, [_p, _i, ctypes.POINTER(_i), _i, ctypes.POINTER(_p)]),
+    "sph_snapshot_wait": (_i, [_p, _pi64]),
+    "sph_snapshot_field": (_i, [_p, _i, ctypes.POINTER(_pd)]),
+    "sph_snapshot_free": (_i, [_p]),
+    "sph_batch_generate_lattice": (_i, [_p, _i, ctypes.POINTER(sph_lattice_spec), _pi64]),
 }
 
 _LIB = None

@@ -15,6 +15,7 @@ import math
 
 import numpy as np
 
+from . import _native as N
 from ._native import lib
 
 _dptr = ctypes.POINTER(ctypes.c_double)
@@ -54,6 +55,26 @@ def jitter(x, dim, amplitude, seed):
     return x
 
 
+def lattice_spec(kind, dim, dr, lo, hi, width=0.0, lid_axis=-1, lid=0.0, type=0.0, rho=0.0,
+                 P=0.0, hydro=None, jitter=0.0, seed=0):
+    """An sph_lattice_spec (include/sph_setup.h) for the device generator.
+    hydro = (axis, rho0, g, H, c) gives the hydrostatic state of
+    examples/collapse_dry.jl:97-100 instead of the constant rho and P."""
+    sp_ = N.sph_lattice_spec()
+    sp_.dim, sp_.kind, sp_.dr = dim, kind, float(dr)
+    for d in range(3):
+        sp_.lo[d], sp_.hi[d] = float(lo[d]), float(hi[d])
+    sp_.width, sp_.lid_axis, sp_.lid = float(width), int(lid_axis), float(lid)
+    sp_.type, sp_.rho, sp_.P = float(type), float(rho), float(P)
+    if hydro is None:
+        sp_.hydro_axis = -1
+    else:
+        sp_.hydro_axis = int(hydro[0])
+        sp_.rho0, sp_.g, sp_.H, sp_.c = (float(v) for v in hydro[1:])
+    sp_.jitter, sp_.seed = float(jitter), int(seed)
+    return sp_
+
+
 class State:
     """Particle arrays (reference order), domain box and physical params."""
 
@@ -63,6 +84,39 @@ class State:
     def params(self):
         return dict(m=self.m, rho0=self.rho0, c=self.c, alpha=self.alpha, hs=self.hs,
                     mu=0.0, nu=0.0, g=tuple(self.g), dt=self.dt, coeff=1.0)
+
+
+def dam_break_specs(dim, dr, R_over_dr=4.8, c=50.0, rho0=1000.0, alpha=0.02, gravity=9.8,
+                    jitter_rel=1e-6, seed=0):
+    """The dam break of dam_break() for the device generator: a State with
+    the geometry and parameters but no particle arrays, and the lattice
+    specs of the fluid and of the walls (generate them in that order)."""
+    if dim == 2:
+        tank = ((0.0, 0.0, 0.0), (4.0, 3.0, 0.0))
+        fluid = ((0.0, 0.0, 0.0), (1.0, 2.0, 0.0))
+        vert = 1
+    else:
+        tank = ((0.0, 0.0, 0.0), (4.0, 1.0, 3.0))
+        fluid = ((0.0, 0.0, 0.0), (1.0, 1.0, 2.0))
+        vert = 2
+    width = 3.5 * dr
+    H = fluid[1][vert]
+    hydro = (vert, rho0, -gravity, H, c)
+    fs = lattice_spec(N.SPH_LATTICE_BOX, dim, dr, fluid[0], fluid[1], type=0.0, hydro=hydro,
+                      jitter=jitter_rel * dr, seed=seed)
+    ws = lattice_spec(N.SPH_LATTICE_WALL_LAYER, dim, dr, tank[0], tank[1], width=width,
+                      lid_axis=vert, lid=tank[1][vert], type=1.0, hydro=hydro)
+    lo = np.array(tank[0], dtype=np.float64)
+    hi = np.array(tank[1], dtype=np.float64)
+    for d in range(dim):
+        lo[d] = tank[0][d] - width
+        hi[d] = tank[1][d] + width
+    g = np.zeros(3)
+    g[vert] = -gravity
+    hs = 2.4 * dr
+    st = State(dim=dim, dr=dr, bbox_lo=lo, bbox_hi=hi, R=R_over_dr * dr, hs=hs,
+               m=rho0 * dr ** dim, rho0=rho0, c=c, alpha=alpha, g=g, dt=0.1 * hs / c)
+    return st, fs, ws
 
 
 def dam_break(dim, dr, R_over_dr=4.8, c=50.0, rho0=1000.0, alpha=0.02, gravity=9.8,

@@ -8,9 +8,14 @@ with the frame number as the time step.  `import_particles` reads such a
 file back into a system, which is how the reference resumes a run
 (examples/cylinder.jl:85).
 
-The fields are downloaded from the device once per frame (one copy per
-field, in the reference's particle order); the encoding runs on the host.
+save_frame downloads the fields synchronously (one copy per field, in the
+reference's particle order).  FrameWriter does the same without stopping the
+steps: it starts an asynchronous device snapshot (sph_snapshot_begin: the
+fields are gathered on the system's stream, so they are the state of the
+call, and copied to pinned memory on a second stream while the next steps
+run) and encodes and writes the file on a worker thread.
 """
+from concurrent.futures import ThreadPoolExecutor
 import os
 import struct
 import xml.etree.ElementTree as ET
@@ -190,6 +195,50 @@ def save_frame(data, sys, *vars, compress=True):
     write_vtp(os.path.join(data.path, name), pts, fields, compress=compress)
     data.frames.append((data.frame, name))
     data.frame += 1
+
+
+class FrameWriter:
+    """save_frame! at frame cadence without blocking the time loop.
+
+    w = FrameWriter(data); w.save(sys, "v", "rho") starts the frame's device
+    snapshot and returns at once; the frame gets the next frame number of
+    `data` now, and its file is written by a worker thread once the copy is
+    done.  flush() waits for every pending frame; close() also stops the
+    workers.  Frames are the same files save_frame writes."""
+
+    def __init__(self, data, workers=2, compress=True):
+        self.data = data
+        self.compress = compress
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.pending = []
+
+    def _write(self, snap, path, vars):
+        try:
+            arrays = snap.wait()
+        finally:
+            snap.close()
+        write_vtp(path, arrays["x"], [(v, arrays[v]) for v in vars], compress=self.compress)
+
+    def save(self, sys, *vars, sid=None):
+        for var in vars:
+            if var not in N.FIELDS:
+                raise KeyError(f"Variable {var} does not exist!")
+        names = ("x",) + tuple(v for v in vars if v != "x")
+        snap = sys.snapshot(*names) if sid is None else sys.snapshot(sid, *names)
+        name = f"frame{self.data.frame}.vtp"
+        self.data.frames.append((self.data.frame, name))
+        self.data.frame += 1
+        self.pending.append(self.pool.submit(self._write, snap, os.path.join(self.data.path, name),
+                                             vars))
+
+    def flush(self):
+        for f in self.pending:
+            f.result()
+        self.pending = []
+
+    def close(self):
+        self.flush()
+        self.pool.shutdown()
 
 
 def save_pvd_file(data):

@@ -144,3 +144,35 @@ def test_header_constants_match_python():
                 assert getattr(N, name) == int(val), name
                 seen += 1
     assert seen >= 10
+
+
+def test_lattice_spec_layout():
+    # int dim, kind; double dr, lo[3], hi[3], width; int lid_axis, hydro_axis;
+    # double lid, type, rho, P, rho0, g, H, c, jitter; uint64_t seed
+    assert ctypes.sizeof(N.sph_lattice_spec) == 160
+    assert N.sph_lattice_spec.lid.offset == 80 and N.sph_lattice_spec.seed.offset == 152
+
+
+def test_jitter_draws_are_counter_based():
+    """The device generator draws jitter value t of the splitmix64 stream as
+    mix(seed + (t + 1) * gamma); the host's sequential sph_jitter must give
+    the same values (the stream is counter-based)."""
+    M = (1 << 64) - 1
+    gamma = 0x9E3779B97F4A7C15
+
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    for dim in (2, 3):
+        n, amp, seed = 50, 1e-6 * 0.02, 7
+        x = np.zeros((n, 3))
+        N.lib().sph_jitter(x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, dim, amp, seed)
+        ref = np.zeros((n, 3))
+        for i in range(n):
+            for d in range(dim):
+                t = i * dim + d
+                u = float(mix((seed + (t + 1) * gamma) & M) >> 11) * 2.0 ** -53
+                ref[i, d] = 0.0 + (2.0 * u - 1.0) * amp
+        assert np.array_equal(x, ref)
