@@ -40,9 +40,13 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "smoothedparticles.jl_amd"))
 
-# Published peaks of one MI355X.
+# Published peaks of one MI355X.  The FP32 vector figure counts packed FP32
+# (two operations per lane per instruction); this build issues no packed
+# FP32, so the rate its kernels can reach is the non-packed one, 78.6
+# TFLOP/s, the same as FP64.
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_FP64_TFLOPS = 78.6
 
 # Algorithmic work of the two pair passes of a step.
@@ -298,6 +302,39 @@ def serve(count=48, rate=2.5, seed=0, budget_gb=32.0):
     return out
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_c0_rates():
+    """The oracle on the 2D dam break (dr = 0.02, 6514 particles, the
+    reference's CPU-runnable case): pair-interactions/s of 200 full steps on
+    one thread and on every host thread."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from smoothedparticles_hip import states
+
+    st = states.dam_break(2, 0.02)
+    out = {}
+    for label, threads in (("1_thread", 1), ("all_threads", O.num_threads())):
+        O.set_threads(threads)
+        o = O.OracleSystem(st.R, st.bbox_lo, st.bbox_hi)
+        o.load(st.x, st.v, st.rho, st.P, st.type)
+        o.create_cell_list()
+        o.density_sum_rho0(True, **st.params())
+        t0 = time.perf_counter()
+        pairs = o.step_wcsph(200, use_rho0=True, **st.params())
+        out[label] = {"value": 2.0 * pairs / (time.perf_counter() - t0), "threads": threads}
+    O.set_threads(out["all_threads"]["threads"])
+    return out
+
+
 def cpu_baseline(st):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -310,6 +347,7 @@ def cpu_baseline(st):
     t0 = time.perf_counter()
     pairs = o.step_wcsph(nsteps, use_rho0=True, **st.params())
     el = time.perf_counter() - t0
+    c0 = cpu_c0_rates()
     # probe sums: the oracle's serial loop over the first 4096 of the bench's probes
     o.create_cell_list()
     P = probe_points(st)[:4096]
@@ -317,6 +355,8 @@ def cpu_baseline(st):
     o.probe_sum(P, st.params()["m"], True)
     el_p = time.perf_counter() - t1
     return dict(value=2.0 * pairs / el, unit="pair-interactions/s", cores=O.num_threads(),
+                cpu_model=cpu_model(), host_threads=os.cpu_count(),
+                c0_2d_dambreak_200_steps=c0,
                 probe_sums_per_s=len(P) / el_p, probe_cores=1,
                 probe_sample=f"{len(P)} of the probe leg's points after the steps, serial C loop, "
                              f"{el_p:.2f} s",
@@ -324,6 +364,45 @@ def cpu_baseline(st):
                 sample=f"{nsteps} full steps (cell list, density, EOS, force, integrate) of the "
                        f"same {len(st.x)}-particle workload by the FP64 C port of the reference "
                        f"algorithm, OpenMP over particles; {el:.1f} s")
+
+
+def accuracy_vs_oracle(dr=0.02, steps=100):
+    """The 3D dam break in the mixed form against the FP64 oracle at the
+    largest size the oracle runs in minutes (dr = 0.02, about 0.5M particles):
+    relative density, velocity and total-energy error every 25 steps."""
+    import numpy as np
+
+    import smoothedparticles_hip as sp
+    from smoothedparticles_hip import states
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    st = states.dam_break(3, dr)
+    p = st.params()
+    g = sp.ParticleSystem(sp.Box(*st.bbox_lo, *st.bbox_hi), st.R, "mixed")
+    g.add_particles(st.x, v=st.v, rho=st.rho, P=st.P, type=st.type)
+    sp.init_reference_density(g, **p)
+    o = O.OracleSystem(st.R, st.bbox_lo, st.bbox_hi)
+    o.load(st.x, st.v, st.rho, st.P, st.type)
+    o.create_cell_list()
+    o.density_sum_rho0(True, **p)
+    out, done = {}, 0
+    for c in range(25, steps + 1, 25):
+        sp.step(g, c - done, scheme=SCHEME, **p)
+        t0 = time.perf_counter()
+        o.step_wcsph(c - done, use_rho0=True, **p)
+        done = c
+        vo = o.field("v")
+        out[str(c)] = dict(
+            rho_rel=float(np.max(np.abs(g.get("rho") / o.field("rho") - 1))),
+            v_rel=float(np.max(np.abs(g.get("v") - vo)) / np.max(np.abs(vo))),
+            energy_rel=float(abs(sp.energy(g, "wcsph_rho0", **p) / o.energy("wcsph_rho0", **p) - 1)),
+            particles=len(g), oracle_s=time.perf_counter() - t0)
+        log(f"vs oracle, step {c}: {out[str(c)]}")
+    return {"workload": f"3D dam break dr={dr}, {len(st.x)} particles, mixed device path vs FP64 "
+                        f"oracle (C port of the reference algorithm), {steps} steps",
+            "errors": out}
 
 
 def main():
@@ -339,6 +418,9 @@ def main():
                     help="skip the mixed-vs-FP64 errors at steps 100 and 1000")
     ap.add_argument("--no-batch", action="store_true", help="skip the Taylor-Green batch")
     ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--accuracy-oracle", action="store_true",
+                    help="also run the mixed 3D dam break (dr=0.02) against the FP64 oracle "
+                         "for 100 steps (several minutes of host time)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -375,20 +457,28 @@ def main():
             name = "force pass (pressure gradient + artificial viscosity over masked pairs)"
         achieved = flops / (ms * 1e-3) / 1e12
         hbm_gbs = bpp * res["n"] / (ms * 1e-3) / 1e9
-        traffic = None
+        traffic, tsrc = None, None
         tpath = os.path.join(ROOT, "profiles", f"{kernel}_pass_{args.workload}_{prec}_traffic.json")
         if os.path.exists(tpath):
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
+            tj = json.load(open(tpath))
+            traffic = tj.get("bytes_per_launch")
+            tsrc = (f"profiles/{os.path.basename(tpath)}: rocprofv3 --pmc FETCH_SIZE and "
+                    f"WRITE_SIZE of this kernel, measured {tj.get('measured', 'earlier')}, "
+                    "not in this run")
         return {
-            "bound": "mfma",
-            "ceiling": "vector ALU: the pair kernels issue no MFMA; peak is the published "
-                       f"{'FP32' if prec == 'mixed' else 'FP64'} rate",
+            "bound": "valu",
+            "ceiling": "vector ALU: the pair kernels issue no MFMA (the field's 'mfma' does not "
+                       "apply); peak is the published "
+                       f"{'packed FP32' if prec == 'mixed' else 'FP64'} vector rate",
             "kernel": name,
             "achieved": achieved,
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
+            "frac_vs_nonpacked_fp32": (achieved / PEAK_FP32_NONPACKED_TFLOPS
+                                       if prec == "mixed" else None),
             "traffic": traffic,
+            "traffic_source": tsrc,
             "flops_per_launch": flops,
             "hbm_algorithmic_GBps": hbm_gbs,
             "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
@@ -432,6 +522,8 @@ def main():
         out["fp64_ms_per_step"] = 1e3 * other["elapsed"] / args.steps
     if not args.no_accuracy:
         out["accuracy_vs_fp64"] = accuracy(st)
+    if args.accuracy_oracle:
+        out["accuracy_vs_oracle"] = accuracy_vs_oracle()
     if not args.no_batch:
         out["tg_batch"] = tg_batch()
     if not args.no_serve:

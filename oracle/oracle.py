@@ -78,6 +78,7 @@ def lib():
             "orc_momentum": (None, [ctypes.c_void_p, _pp, _pd]),
             "orc_step_wcsph": (ctypes.c_int64, [ctypes.c_void_p, _pp, ctypes.c_int, ctypes.c_int]),
             "orc_num_threads": (ctypes.c_int, []),
+            "orc_set_threads": (None, [ctypes.c_int]),
             "orc_wendland2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "orc_rDwendland2": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
             "orc_wendland3": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
@@ -308,6 +309,10 @@ def brute_pairs(x, h):
 
 def mix64(z):
     return lib().orc_mix64(int(z))
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 def num_threads():

@@ -1010,6 +1010,15 @@ int64_t orc_step_collapse_dry(orc_sys* s, const orc_params* pr, int nsteps)
     return pairs;
 }
 
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP
