@@ -183,14 +183,15 @@ class Server:
                                   sum(x.footprint() for x in self.batches.values()))
 
     def admit(self, r, now):
+        cost = self.cost(r)  # the charge the admission test used
         st = r.build()
         b = self._batch(r.dim)
         r.sid = b.add_state(st, periodic=r.periodic)
         r.admitted = now
         self.active[r.dim].append(r)
-        self._measure(r.dim)
-        self.charge[r.rid] = self.cost(r)
-        self.in_use += self.charge[r.rid]
+        self._measure(r.dim)  # the charge of later requests
+        self.charge[r.rid] = cost
+        self.in_use += cost
         self.peak_in_use = max(self.peak_in_use, self.in_use)
 
     def _advance(self, dim):
