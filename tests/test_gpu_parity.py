@@ -329,12 +329,11 @@ def test_autotune_keeps_state():
         assert np.array_equal(g.get(f), a)
 
 
-def test_fp64_operators_after_mixed_density_pass():
-    """In the mixed mode the operators without a mixed form run FP64 kernels;
-    after a mixed density pass they find the same pairs whether they read its
-    masks (with or without the kept-candidate lists) or re-test.  The FP64
-    and FP32 culls keep different candidates, so reading the mixed pass's
-    chunks sums in another order: equal to FP64 rounding, not bit for bit."""
+def test_operators_after_mixed_density_pass():
+    """After a mixed density pass the rho0^2 force and balance_of_mass! (in
+    their mixed forms) find the same pairs whether they read its masks (with
+    or without the kept-candidate lists) or re-test, and add them in the same
+    runs: equal to 1e-12 of the largest value."""
     n = 2500
     x, lo, hi, rng = random_cloud(n, 2, 57)
     v, rho, P, typ = fields(rng, n, 2)
