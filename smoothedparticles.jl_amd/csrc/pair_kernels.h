@@ -454,18 +454,22 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int
     tt = -1;
     r0 = cnt = first_run = nruns = 0;
     if (tot == 0) return 0;
+    // The run length B is the smallest with at most 64 runs.  The number of
+    // runs does not grow with B, is at most 64 at B = max(pop) (one run per
+    // lane with pairs) and cannot be at most 64 below ceil(tot / 64).
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
     int m = run_magic(B);
-    int need = wave_sum7(runs_of(pop, B, m));
-    if (need > 64) {
-        ++B;
-        m = B <= 64 ? run_magic(B) : 0;
-        need = wave_sum7(runs_of(pop, B, m));
-        if (need > 64) {
-            const int nz = __popcll(__ballot(pop > 0));
-            B = nz < 64 ? (tot + (64 - nz) - 1) / (64 - nz) : wave_max(pop);
-            m = B <= 64 ? run_magic(B) : 0;
+    if (wave_sum7(runs_of(pop, B, m)) > 64) {
+        int lo = B + 1, hi = wave_max(pop);
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (wave_sum7(runs_of(pop, mid, run_magic(mid))) > 64)
+                lo = mid + 1;
+            else
+                hi = mid;
         }
+        B = lo;
+        m = run_magic(B);
     }
     nruns = runs_of(pop, B, m);
     first_run = lanes_below_sum7(nruns);
