@@ -59,7 +59,7 @@ PEAK_FP64_TFLOPS = 78.6
 #     candidate of the 27 (9) cells; cand_all reports that count beside.
 #   force pass: only the accepted pairs of fluid targets (the masks of the
 #     density pass say which), about 45 flops each (pressure gradient, Monaghan
-#     viscosity, compensated sums); bytes: x, v, rho, P/rho^2, type read and a
+#     viscosity, FP64 accumulation); bytes: x, v, rho, P/rho^2, type read and a
 #     written once.
 FLOPS_PER_CANDIDATE = 9.0
 FLOPS_PER_DENSITY_PAIR = 15.0

@@ -49,7 +49,7 @@ typedef struct sph_system sph_system;
 
 /* precision modes */
 #define SPH_FP64 0   /* plain FP64 everywhere (the drop-in) */
-#define SPH_MIXED 1  /* FP32 relative-coordinate pair arithmetic with compensated
+#define SPH_MIXED 1  /* FP32 relative-coordinate pair arithmetic with FP64
                         accumulation; FP64 positions, state and integrator */
 
 /* fields (sph_get_field / sph_set_field) */

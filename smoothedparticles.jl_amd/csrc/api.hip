@@ -1346,7 +1346,7 @@ int neighbour_digest(sph_system* s, int slot, int64_t* counts, uint64_t* hashes,
 // Mixed precision: times the density and force passes for every tile size on
 // the current cell lists with a zero mass (both passes then add exactly zero
 // to rho and a, so the state is left as it was) and keeps the fastest.  The
-// mixed sums are folded per run in FP32 with 2Sum, so their bits depend on T
+// mixed sums are plain FP32 per run before the FP64 fold, so their bits depend on T
 // as well; the mixed path is stated by tolerance.  FP64: the occupancy rule.
 int autotune(sph_system* s, int* best)
 {

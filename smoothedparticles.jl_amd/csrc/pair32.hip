@@ -1,5 +1,5 @@
 // Mixed-precision pair kernels: FP32 pair arithmetic on tile-relative
-// coordinates with compensated sums; operators, tile loop, dispatch.
+// coordinates with FP64 accumulators; operators, tile loop, dispatch.
 #include "pair_kernels.h"
 
 namespace sph {
@@ -7,8 +7,9 @@ namespace sph {
 // =============================================================== mixed ====
 // FP32 pair arithmetic on coordinates relative to the tile origin (the FP64
 // position of the tile's first target), converted while staging.  Per-lane
-// partial sums are plain FP32 within a 64-candidate chunk and folded into a
-// compensated (sum, error) pair after every chunk.
+// partial sums are plain FP32 within a run (the pairs of one target in one
+// 64-candidate chunk) and added to an FP64 accumulator after every chunk
+// (two instructions per field, where a 2Sum in FP32 takes six).
 
 // The hardware square root (v_sqrt_f32, within 1 ulp) instead of the
 // correctly rounded sequence the compiler emits for sqrtf (about 15
@@ -49,7 +50,7 @@ struct Density32 {
         float w;
     };
     struct Acc {
-        float s, e;
+        double s;
     };
     static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
                                 double oz, Tgt& t)
@@ -72,7 +73,7 @@ struct Density32 {
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
-    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
+    static __device__ void init(Acc& a) { a.s = 0.0; }
     static __device__ void zero(Part& q) { q.w = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
@@ -81,16 +82,16 @@ struct Density32 {
                                     int, int, int, float, float, float, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
-        const float x = r * (float)d.g.inv_h;
-        const float u = 1.0f - x;
+        const float ih = (float)d.g.inv_h;
+        const float u = fmaf(-r, ih, 1.0f);
         const float u2 = u * u;
-        q.w = fmaf(u2 * u2, fmaf(4.0f, x, 1.0f), q.w);
+        q.w = fmaf(u2 * u2, fmaf(4.0f * ih, r, 1.0f), q.w);
     }
-    static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.w); }
+    static __device__ void fold(Acc& a, const Part& q) { a.s += (double)q.w; }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int out_col)
     {
-        double w = (double)a.s + (double)a.e;
+        double w = a.s;
         for (int o = T; o < 64; o <<= 1) w += __shfl_xor(w, o);
         if (!write) return;
         double v = c.overwrite ? 0.0 : COL(c, out_col, p);
@@ -114,7 +115,7 @@ struct Force32 {
         float ax, ay, az, lx, ly, lz;
     };
     struct Acc {
-        float sx, sy, sz, ex, ey, ez, lx, ly, lz, fx, fy, fz;
+        double ax, ay, az, lx, ly, lz;
     };
     static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
                                 double oz, Tgt& t)
@@ -147,11 +148,7 @@ struct Force32 {
         s.pos[l] = a;
         s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, (float)r.a0);
     }
-    static __device__ void init(Acc& a)
-    {
-        a.sx = a.sy = a.sz = a.ex = a.ey = a.ez = 0.0f;
-        a.lx = a.ly = a.lz = a.fx = a.fy = a.fz = 0.0f;
-    }
+    static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
     // variant bit 0: Monaghan viscosity (avk != 0); bit 1: laminar (mu != 0)
     static constexpr int NVAR = 4;
@@ -165,7 +162,7 @@ struct Force32 {
                                     float dz, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
-        const float u = 1.0f - r * (float)d.g.inv_h;
+        const float u = fmaf(-r, (float)d.g.inv_h, 1.0f);
         const float u3 = u * u * u;
         const float4 v = s.v[jj];
         float A = t.pr + s.pos[jj].w;
@@ -187,20 +184,13 @@ struct Force32 {
     }
     static __device__ void fold(Acc& a, const Part& q)
     {
-        two_sum_add(a.sx, a.ex, q.ax);
-        two_sum_add(a.sy, a.ey, q.ay);
-        two_sum_add(a.sz, a.ez, q.az);
-        two_sum_add(a.lx, a.fx, q.lx);
-        two_sum_add(a.ly, a.fy, q.ly);
-        two_sum_add(a.lz, a.fz, q.lz);
+        a.ax += (double)q.ax; a.ay += (double)q.ay; a.az += (double)q.az;
+        a.lx += (double)q.lx; a.ly += (double)q.ly; a.lz += (double)q.lz;
     }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
     {
-        double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
-               az = (double)a.sz + (double)a.ez;
-        double lx = (double)a.lx + (double)a.fx, ly = (double)a.ly + (double)a.fy,
-               lz = (double)a.lz + (double)a.fz;
+        double ax = a.ax, ay = a.ay, az = a.az, lx = a.lx, ly = a.ly, lz = a.lz;
         for (int o = T; o < 64; o <<= 1) {
             ax += __shfl_xor(ax, o); ay += __shfl_xor(ay, o); az += __shfl_xor(az, o);
             lx += __shfl_xor(lx, o); ly += __shfl_xor(ly, o); lz += __shfl_xor(lz, o);
@@ -232,7 +222,7 @@ struct ForceRho032 {
         float ax, ay, az;
     };
     struct Acc {
-        float sx, sy, sz, ex, ey, ez;
+        double ax, ay, az;
     };
     static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
                                 double oz, Tgt& t)
@@ -259,7 +249,7 @@ struct ForceRho032 {
         a.w = (float)r.a0;
         s.pos[l] = a;
     }
-    static __device__ void init(Acc& a) { a.sx = a.sy = a.sz = a.ex = a.ey = a.ez = 0.0f; }
+    static __device__ void init(Acc& a) { a.ax = a.ay = a.az = 0.0; }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
@@ -269,21 +259,18 @@ struct ForceRho032 {
                                     float dz, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
-        const float u = 1.0f - r * (float)d.g.inv_h;
+        const float u = fmaf(-r, (float)d.g.inv_h, 1.0f);
         const float f = u * u * u * (t.P + s.pos[jj].w);
         q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
     }
     static __device__ void fold(Acc& a, const Part& q)
     {
-        two_sum_add(a.sx, a.ex, q.ax);
-        two_sum_add(a.sy, a.ey, q.ay);
-        two_sum_add(a.sz, a.ez, q.az);
+        a.ax += (double)q.ax; a.ay += (double)q.ay; a.az += (double)q.az;
     }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
     {
-        double ax = (double)a.sx + (double)a.ex, ay = (double)a.sy + (double)a.ey,
-               az = (double)a.sz + (double)a.ez;
+        double ax = a.ax, ay = a.ay, az = a.az;
         for (int o = T; o < 64; o <<= 1) {
             ax += __shfl_xor(ax, o); ay += __shfl_xor(ay, o); az += __shfl_xor(az, o);
         }
@@ -314,7 +301,7 @@ struct Continuity32 {
         float d;
     };
     struct Acc {
-        float s, e;
+        double s;
     };
     static __device__ void load(const PairCtx& c, const SysDesc& d, int p, double ox, double oy,
                                 double oz, Tgt& t)
@@ -344,7 +331,7 @@ struct Continuity32 {
         s.pos[l] = a;
         s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, 0.0f);
     }
-    static __device__ void init(Acc& a) { a.s = a.e = 0.0f; }
+    static __device__ void init(Acc& a) { a.s = 0.0; }
     static __device__ void zero(Part& q) { q.d = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
@@ -354,17 +341,17 @@ struct Continuity32 {
                                     float dz, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
-        const float u = 1.0f - r * (float)d.g.inv_h;
+        const float u = fmaf(-r, (float)d.g.inv_h, 1.0f);
         const float4 v = s.v[jj];
         const float xv = fmaf(dz, t.vz - v.z, fmaf(dy, t.vy - v.y, dx * (t.vx - v.x)));
         const float dd = fmaf(2.0f * (float)d.ph.nu, t.dr - s.pos[jj].w, xv);
         q.d = fmaf(u * u * u, dd, q.d);
     }
-    static __device__ void fold(Acc& a, const Part& q) { two_sum_add(a.s, a.e, q.d); }
+    static __device__ void fold(Acc& a, const Part& q) { a.s += (double)q.d; }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
     {
-        double v = (double)a.s + (double)a.e;
+        double v = a.s;
         for (int o = T; o < 64; o <<= 1) v += __shfl_xor(v, o);
         if (!write) return;
         const double b = c.overwrite ? 0.0 : COL(c, C_DRHO, p);
@@ -671,9 +658,25 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 if (s != 0) word = 0;
             }
             if (s == 0) bal.w[t] = word;
+            bool own;
             int tt, r0, cnt, frun, nrun;
-            const int tot = split_runs(lane, __popcll(word), bal, tt, r0, cnt, frun, nrun);
-            if (tot > 0) {
+            const int tot = split_runs(lane, __popcll(word), bal, own, tt, r0, cnt, frun, nrun);
+            if (own) {
+                // every target's pairs on its owner lane
+                typename Op::Part part;
+                Op::zero(part);
+                const int n = __popcll(word);
+                if (Op::INTERACT && n > 0) {
+                    const int hp = first + t;
+                    switch (Op::variant(d)) {
+                    case 0: walk32<0, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    case 1: walk32<1, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    case 2: walk32<2, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    default: walk32<3, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    }
+                }
+                if (n > 0) Op::fold(acc, part);
+            } else if (tot > 0) {
                 typename Op::Part part;
                 Op::zero(part);
                 // the run's target record from its owner lane (every lane

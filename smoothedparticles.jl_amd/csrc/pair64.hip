@@ -549,9 +549,25 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
             const int pop = __popcll(word);
             npairs += (unsigned)pop;
             if (s == 0) bal.w[t] = word;
+            bool own;
             int tt, r0, cnt, frun, nrun;
-            const int tot = split_runs(lane, pop, bal, tt, r0, cnt, frun, nrun);
-            if (tot > 0) {
+            const int tot = split_runs(lane, pop, bal, own, tt, r0, cnt, frun, nrun);
+            if (own) {
+                // every target's pairs on its owner lane
+                typename Op::Part part;
+                Op::zero(part);
+                const int n = __popcll(word);
+                if (Op::INTERACT && n > 0) {
+                    const int hp = first + t;
+                    switch (Op::variant(d)) {
+                    case 0: walk64<0, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    case 1: walk64<1, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    case 2: walk64<2, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    default: walk64<3, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
+                    }
+                }
+                if (n > 0) Op::fold(acc, part);
+            } else if (tot > 0) {
                 typename Op::Part part;
                 Op::zero(part);
                 const typename Op::Tgt ht = shfl_value(tg, tt < 0 ? 0 : tt);

@@ -86,15 +86,6 @@ __device__ inline void count_tests(const PairCtx& c, unsigned long long n)
     if (c.test_counter && __lane_id() == 0 && n) atomicAdd(c.test_counter, n);
 }
 
-// Compensated (2Sum) addition of a chunk partial into (s, e).
-__device__ inline void two_sum_add(float& s, float& e, float x)
-{
-    const float t = s + x;
-    const float bp = t - s;
-    e += (s - (t - bp)) + (x - bp);
-    s = t;
-}
-
 // Tile queues.  The tile range [0, ntiles) is cut into NQUEUE contiguous
 // parts, one per queue head.  A wave starts on queue blockIdx % NQUEUE (the
 // XCD it runs on, so neighbouring tiles share that XCD's L2), takes qgrab
@@ -443,24 +434,33 @@ __device__ inline int runs_of(int pop, int B, int m)
 }
 
 // Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
-// only, 0 elsewhere).  Returns the total; sets the lane's run (target tt,
-// first rank r0, count cnt) and, for owner lanes, the first run and the
-// number of runs of their target.
+// only, 0 elsewhere).  Returns the total.  When the run length is the
+// largest pop, every target is one run and the runs need no moving: own is
+// set and each owner lane walks its own target (tt, r0, cnt are then not
+// set).  That is always so when all 64 lanes hold a target with pairs, since
+// a shorter run length splits the largest and needs 65 runs.  Otherwise
+// sets the lane's run (target tt, first rank r0, count cnt) and, for owner
+// lanes, the first run and the number of runs of their target.
 template <class Part>
-__device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int& r0, int& cnt,
-                                 int& first_run, int& nruns)
+__device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, int& tt, int& r0,
+                                 int& cnt, int& first_run, int& nruns)
 {
     const int tot = wave_sum7(pop);
+    own = false;
     tt = -1;
     r0 = cnt = first_run = nruns = 0;
     if (tot == 0) return 0;
+    if (__ballot(pop > 0) == ~0ull) {
+        own = true;
+        return tot;
+    }
     // The run length B is the smallest with at most 64 runs.  The number of
     // runs does not grow with B, is at most 64 at B = max(pop) (one run per
     // lane with pairs) and cannot be at most 64 below ceil(tot / 64).
+    const int maxp = wave_max(pop);
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
-    int m = run_magic(B);
-    if (wave_sum7(runs_of(pop, B, m)) > 64) {
-        int lo = B + 1, hi = wave_max(pop);
+    if (B < maxp && wave_sum7(runs_of(pop, B, run_magic(B))) > 64) {
+        int lo = B + 1, hi = maxp;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (wave_sum7(runs_of(pop, mid, run_magic(mid))) > 64)
@@ -469,8 +469,12 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, int& tt, int
                 hi = mid;
         }
         B = lo;
-        m = run_magic(B);
     }
+    if (B >= maxp) {
+        own = true;
+        return tot;
+    }
+    const int m = run_magic(B);
     nruns = runs_of(pop, B, m);
     first_run = lanes_below_sum7(nruns);
     bal.lane_t[lane] = -1;

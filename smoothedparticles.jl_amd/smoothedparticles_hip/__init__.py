@@ -179,7 +179,7 @@ class ParticleSystem:
     ``domain`` is a Box (only its bounding box matters, as in the
     reference); ``h`` is the neighbour radius and cell size; ``precision``
     is "fp64" (plain FP64, the drop-in) or "mixed" (FP32 pair arithmetic on
-    relative coordinates with compensated sums, FP64 state).
+    relative coordinates with FP64 accumulation, FP64 state).
     """
 
     def __init__(self, domain, h, precision="fp64"):

@@ -206,7 +206,7 @@ def test_internal_force(precision, tol, dim, mu):
 def test_pressure_force_rho0_and_continuity(precision, tol, dim):
     """internal_force! with rho0^2 (tests/test_collision_2d.jl:75-78) and
     balance_of_mass! (examples/collapse_dry.jl:112-115); the mixed forms are
-    FP32 pair arithmetic with compensated sums: within 1e-5 of max|a| and
+    FP32 pair arithmetic with FP64 accumulation: within 1e-5 of max|a| and
     max|Drho|, FP64 within 1e-11."""
     n = 2500 if dim == 2 else 4000
     x, lo, hi, rng = random_cloud(n, dim, 41)
