@@ -107,17 +107,24 @@ struct Force32 {
     static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
     static constexpr int WAVES = 5;  // spills a few registers; LDS fits five blocks too
     static constexpr int WAVES_KEPT = 5;
+    // rho is staged as rho * 0.5 / avk (target and candidates), so the
+    // viscous term's (avk vx) / (0.5 (rho_p + rho_q) (r^2 + eps)) is
+    // vx / ((rh_p + rh_q) (r^2 + eps))
     struct Tgt {
-        float x, y, z, vx, vy, vz, pr, rho;
+        float x, y, z, vx, vy, vz, pr, rh;
         bool fluid;
     };
+    static __device__ double rho_scale(const SysDesc& d)
+    {
+        return d.ph.avk != 0.0 ? 0.5 / d.ph.avk : 0.0;
+    }
     struct Part {
         float ax, ay, az, lx, ly, lz;
     };
     struct Acc {
         double ax, ay, az, lx, ly, lz;
     };
-    static __device__ void load(const PairCtx& c, const SysDesc&, int p, double ox, double oy,
+    static __device__ void load(const PairCtx& c, const SysDesc& d, int p, double ox, double oy,
                                 double oz, Tgt& t)
     {
         t.x = (float)(COL(c, C_X, p) - ox);
@@ -125,7 +132,7 @@ struct Force32 {
         t.z = (float)(COL(c, C_Z, p) - oz);
         t.vx = (float)COL(c, C_VX, p); t.vy = (float)COL(c, C_VY, p); t.vz = (float)COL(c, C_VZ, p);
         t.pr = (float)c.prho2[p];
-        t.rho = (float)COL(c, C_RHO, p);
+        t.rh = (float)(COL(c, C_RHO, p) * rho_scale(d));
         t.fluid = COL(c, C_TYPE, p) == 0.0;
     }
     static __device__ bool active(const Tgt& t) { return t.fluid; }
@@ -140,13 +147,13 @@ struct Force32 {
         return r;
     }
     template <bool FLAT>
-    static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
+    static __device__ void put(const Raw& r, int l, const SysDesc& d, double ox, double oy,
                                     double oz, Lds& s)
     {
         float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
         a.w = (float)r.a4;
         s.pos[l] = a;
-        s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, (float)r.a0);
+        s.v[l] = make_float4((float)r.a1, (float)r.a2, (float)r.a3, (float)(r.a0 * rho_scale(d)));
     }
     static __device__ void init(Acc& a) { a.ax = a.ay = a.az = a.lx = a.ly = a.lz = 0.0; }
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = q.lx = q.ly = q.lz = 0.0f; }
@@ -161,19 +168,18 @@ struct Force32 {
                                     const Lds& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
+        const float4 v = s.v[jj];
         const float r = sqrt_hw(ss);
         const float u = fmaf(-r, (float)d.g.inv_h, 1.0f);
         const float u3 = u * u * u;
-        const float4 v = s.v[jj];
         float A = t.pr + s.pos[jj].w;
         const float dvx = t.vx - v.x, dvy = t.vy - v.y, dvz = t.vz - v.z;
         if (V & 1) {
             // evaluated for every pair and selected, so the unrolled pair loop
             // stays one basic block
             const float vx = fmaf(dvz, dz, fmaf(dvy, dy, dvx * dx));
-            const float rhobar = 0.5f * (t.rho + v.w);
-            const float den = rhobar * (ss + (float)d.ph.eps);
-            const float Av = fmaf((float)d.ph.avk * vx, __builtin_amdgcn_rcpf(den), A);
+            const float den = (t.rh + v.w) * (ss + (float)d.ph.eps);
+            const float Av = fmaf(vx, __builtin_amdgcn_rcpf(den), A);
             A = vx < 0.0f ? Av : A;
         }
         const float f = u3 * A;
