@@ -210,8 +210,12 @@ __global__ __launch_bounds__(256) void k_scatter(const int* meta, const int* __r
 //   - a block whose threads all lie in one bucket of more than RANK_LDS_MIN
 //     particles (a removal bucket after many particles left the box) counts
 //     against the bucket staged through LDS, 1024 indices at a time;
-//   - otherwise each thread scans its bucket (a cell: tens of particles).
+//   - otherwise the buckets the block's threads lie in (a few cells of tens
+//     to hundreds of particles) are staged through LDS when they fit in
+//     RANK_SPAN entries, and each thread counts against its bucket there;
+//   - else each thread scans its bucket in global memory.
 static constexpr int RANK_LDS_MIN = 512;
+static constexpr int RANK_SPAN = 2048;
 
 __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __restrict__ cell_start,
                                               const int* __restrict__ tmp_idx,
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
                                               const int* __restrict__ tmp_ref, int* perm,
                                               int dead_key)
 {
-    __shared__ int tile[1024];
+    __shared__ int tile[RANK_SPAN];
     const int n = meta[META_NTOT];
     const int b0 = blockIdx.x * 256;
     if (b0 >= n) return;  // whole block past the end (uniform)
@@ -242,6 +246,25 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
             }
         }
         if (in) perm[s0 + rank] = tmp_idx[u];
+        return;
+    }
+    // the span of the buckets of the block's first and last thread holds
+    // every bucket in between (scatter order is bucket order)
+    const int span0 = s0, span1 = cell_start[c1 + 1];
+    if (c1 != dead_key && span1 - span0 <= RANK_SPAN) {
+        for (int j = threadIdx.x; j < span1 - span0; j += 256) tile[j] = tmp_ref[span0 + j];
+        __syncthreads();
+        if (!in) return;
+        const int c = tmp_key[u];
+        const int s = cell_start[c], e = cell_start[c + 1];
+        const int r = tile[u - span0];
+        int rank = 0;
+#pragma unroll 8
+        for (int v = s; v < e; ++v) {
+            const int rv = tile[v - span0];
+            rank += rv > r || (rv == r && v < u);
+        }
+        perm[s + rank] = tmp_idx[u];
         return;
     }
     if (!in) return;
