@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
                                               const int* __restrict__ tmp_ref, int* perm,
                                               int dead_key)
 {
-    __shared__ int tile[RANK_SPAN];
+    __shared__ __align__(16) int tile[RANK_SPAN];
     const int n = meta[META_NTOT];
     const int b0 = blockIdx.x * 256;
     if (b0 >= n) return;  // whole block past the end (uniform)
@@ -255,15 +255,20 @@ __global__ __launch_bounds__(256) void k_rank(const int* meta, const int* __rest
         for (int j = threadIdx.x; j < span1 - span0; j += 256) tile[j] = tmp_ref[span0 + j];
         __syncthreads();
         if (!in) return;
+        // a live bucket (a cell or a system's removal bucket) holds one
+        // system's particles, whose reference indices are distinct: the
+        // rank is the count of larger ones, read four at a time
         const int c = tmp_key[u];
         const int s = cell_start[c], e = cell_start[c + 1];
         const int r = tile[u - span0];
-        int rank = 0;
-#pragma unroll 8
-        for (int v = s; v < e; ++v) {
-            const int rv = tile[v - span0];
-            rank += rv > r || (rv == r && v < u);
+        int rank = 0, v = s - span0;
+        const int ve = e - span0;
+        for (; v < ve && (v & 3); ++v) rank += tile[v] > r;
+        for (; v + 4 <= ve; v += 4) {
+            const int4 q = *reinterpret_cast<const int4*>(&tile[v]);
+            rank += (q.x > r) + (q.y > r) + (q.z > r) + (q.w > r);
         }
+        for (; v < ve; ++v) rank += tile[v] > r;
         perm[s + rank] = tmp_idx[u];
         return;
     }
@@ -424,6 +429,44 @@ __global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ til
     const int lane = threadIdx.x & 63;
     const int t = lane & (T - 1);
     const long long nwaves = (long long)gridDim.x * 4;
+    if (T == 64) {
+        // one tile per wave: the coordinates go through LDS, lane 8 c + j
+        // (c < 6: min x, max x, min y, ...) reduces eight of them and the
+        // eight lanes of c combine theirs in three shuffle steps
+        __shared__ double xs[4][3][64];
+        double(*w)[64] = xs[threadIdx.x >> 6];
+        const int c = lane >> 3, j = lane & 7, axis = c >> 1;
+        const bool lo = (c & 1) == 0;
+        // the block's four waves step together (block barriers)
+        for (long long b = (long long)blockIdx.x * 4; b < ntiles; b += nwaves) {
+            const long long i = b + (threadIdx.x >> 6);
+            if (i < ntiles) {
+                const int4 td = tiles[i];
+                const int p = td.y + lane;
+                // targets past the cell's end take the first target's place
+                const int pp = p < cell_start[td.x + 1] ? p : td.y;
+                w[0][lane] = x[pp];
+                w[1][lane] = y[pp];
+                w[2][lane] = FLAT ? 0.0 : z[pp];
+            }
+            __syncthreads();
+            double v = 0.0;
+            if (c < 6) {
+                v = w[axis][8 * j];
+                for (int k = 1; k < 8; ++k) {
+                    const double u = w[axis][8 * j + k];
+                    v = lo ? fmin(v, u) : fmax(v, u);
+                }
+            }
+            for (int o = 1; o < 8; o <<= 1) {
+                const double u = __shfl_xor(v, o);
+                v = lo ? fmin(v, u) : fmax(v, u);
+            }
+            if (i < ntiles && c < 6 && j == 0) boxes[6 * i + c] = v;
+            __syncthreads();
+        }
+        return;
+    }
     for (long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
          wave * (64 / T) < ntiles; wave += nwaves) {
         const long long i = wave * (64 / T) + lane / T;
