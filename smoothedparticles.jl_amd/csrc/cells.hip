@@ -73,6 +73,50 @@ __global__ __launch_bounds__(256) void k_sys_prep(int nsys, int* sc, int* meta)
 }
 
 // ------------------------------------------------------------------ keys --
+// Bucket key of particle i of system s: its cell, the system's removal
+// bucket (outside the box, or not finite in a periodic system) or the dead
+// bucket (retired system).  Periodic systems wrap the position (written
+// back).
+__device__ inline int key_of(const SysDesc& d, int s, int nsys, int cells_total, double* x,
+                             double* y, double* z, int i)
+{
+    const Geo& g = d.g;
+    double px = x[i], py = y[i], pz = z[i];
+    if (!d.live) return cells_total + nsys;
+    if (g.periodic) {
+        // positions live in [bmin, bmin + box) on the periodic axes
+        if (!(isfinite(px) && isfinite(py) && isfinite(pz))) return cells_total + s;
+        px = px - g.box[0] * floor((px - g.bmin[0]) / g.box[0]);
+        py = py - g.box[1] * floor((py - g.bmin[1]) / g.box[1]);
+        x[i] = px;
+        y[i] = py;
+        long long ci = (long long)floor((px - g.bmin[0]) / g.cell[0]);
+        long long cj = (long long)floor((py - g.bmin[1]) / g.cell[1]);
+        long long ck = 0;
+        if (g.L3 > 1) {
+            pz = pz - g.box[2] * floor((pz - g.bmin[2]) / g.box[2]);
+            z[i] = pz;
+            ck = (long long)floor((pz - g.bmin[2]) / g.cell[2]);
+        }
+        ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
+        cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
+        ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
+        return d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+    }
+    const bool inside = g.bmin[0] <= px && px <= g.bmax[0] && g.bmin[1] <= py &&
+                        py <= g.bmax[1] && g.bmin[2] <= pz && pz <= g.bmax[2];
+    if (!inside) return cells_total + s;
+    long long ci = (long long)floor(px / g.h) - g.kp[0];
+    long long cj = (long long)floor(py / g.h) - g.kp[1];
+    long long ck = (long long)floor(pz / g.h) - g.kp[2];
+    // In range by monotonicity of the correctly rounded division; the clamp
+    // only guards memory.
+    ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
+    cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
+    ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
+    return d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
+}
+
 __global__ __launch_bounds__(256) void k_keys(const SysDesc* __restrict__ sysd, int nsys,
                                               int cells_total, double* x, double* y, double* z,
                                               const int* __restrict__ sysid, const int* meta,
@@ -81,55 +125,17 @@ __global__ __launch_bounds__(256) void k_keys(const SysDesc* __restrict__ sysd, 
 {
     const int n = meta[META_N];
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int s = i < n ? sysid[i] : 0;
+    // a wave of one system reads its descriptor once, through the scalar
+    // cache (the usual case: systems are contiguous)
+    const int s0 = __builtin_amdgcn_readfirstlane(s);
     int k = -1;
-    if (i < n) {
-        const int s = sysid[i];
-        const SysDesc& d = sysd[s];
-        const Geo& g = d.g;
-        double px = x[i], py = y[i], pz = z[i];
-        if (!d.live) {
-            k = cells_total + nsys;
-        } else if (g.periodic) {
-            // positions live in [bmin, bmin + box) on the periodic axes
-            if (!(isfinite(px) && isfinite(py) && isfinite(pz))) {
-                k = cells_total + s;
-            } else {
-                px = px - g.box[0] * floor((px - g.bmin[0]) / g.box[0]);
-                py = py - g.box[1] * floor((py - g.bmin[1]) / g.box[1]);
-                x[i] = px;
-                y[i] = py;
-                long long ci = (long long)floor((px - g.bmin[0]) / g.cell[0]);
-                long long cj = (long long)floor((py - g.bmin[1]) / g.cell[1]);
-                long long ck = 0;
-                if (g.L3 > 1) {
-                    pz = pz - g.box[2] * floor((pz - g.bmin[2]) / g.box[2]);
-                    z[i] = pz;
-                    ck = (long long)floor((pz - g.bmin[2]) / g.cell[2]);
-                }
-                ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
-                cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
-                ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
-                k = d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
-            }
-        } else {
-            const bool inside = g.bmin[0] <= px && px <= g.bmax[0] && g.bmin[1] <= py &&
-                                py <= g.bmax[1] && g.bmin[2] <= pz && pz <= g.bmax[2];
-            if (inside) {
-                long long ci = (long long)floor(px / g.h) - g.kp[0];
-                long long cj = (long long)floor(py / g.h) - g.kp[1];
-                long long ck = (long long)floor(pz / g.h) - g.kp[2];
-                // In range by monotonicity of the correctly rounded division;
-                // the clamp only guards memory.
-                ci = ci < 0 ? 0 : (ci >= g.L1 ? g.L1 - 1 : ci);
-                cj = cj < 0 ? 0 : (cj >= g.L2 ? g.L2 - 1 : cj);
-                ck = ck < 0 ? 0 : (ck >= g.L3 ? g.L3 - 1 : ck);
-                k = d.cell_base + (int)(ci + (long long)g.L1 * (cj + (long long)g.L2 * ck));
-            } else {
-                k = cells_total + s;
-            }
-        }
-        key[i] = k;
+    if (__all(i >= n || s == s0)) {
+        if (i < n) k = key_of(sysd[s0], s0, nsys, cells_total, x, y, z, i);
+    } else if (i < n) {
+        k = key_of(sysd[s], s, nsys, cells_total, x, y, z, i);
     }
+    if (i < n) key[i] = k;
     // One atomic per distinct key among the wave's lanes.
     const int lane = __lane_id();
     unsigned long long active = __ballot(i < n);
@@ -402,9 +408,12 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
         // 64 kept-candidate entries per chunk of a tile, T mask words
         meta[META_KEPT_OK] = (long long)(need / T * 64) <= kept_cap ? 1 : 0;
     }
+    const int t0 = c < ncells ? tile_start[c] : 0;
+    const int nt = c < ncells ? tile_start[c + 1] - t0 : 0;
+    // cells with tiles, one atomic per wave
+    const unsigned long long wt = __ballot(nt > 0);
+    if (__lane_id() == 0 && wt) atomicAdd(&meta[META_NCELLS], __popcll(wt));
     if (c >= ncells) return;
-    const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
-    if (nt > 0) atomicAdd(&meta[META_NCELLS], 1);
     const int cs = cell_start[c];
     const long long words = (long long)chunks[c] * T;
     for (int t = 0; t < nt; ++t) {
