@@ -39,6 +39,7 @@ __device__ inline float4 rel_pos32(const Raw& r, double ox, double oy, double oz
 
 struct Density32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;  // pos.w = |x|^2
     static constexpr int WAVES = 6;  // a few spilled registers outside the loops
@@ -103,10 +104,11 @@ struct Density32 {
 
 struct Force32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     using Lds = Lds32Full;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
     static constexpr int WAVES = 5;  // spills a few registers; LDS fits five blocks too
-    static constexpr int WAVES_KEPT = 5;
+    static constexpr int WAVES_KEPT = 6;
     // rho is staged as rho * 0.5 / avk (target and candidates), so the
     // viscous term's (avk vx) / (0.5 (rho_p + rho_q) (r^2 + eps)) is
     // vx / ((rh_p + rh_q) (r^2 + eps))
@@ -217,6 +219,7 @@ struct Force32 {
 // a += -kdW/rho0^2 u^3 (P_p + P_q) x_pq for every target, P staged in FP32.
 struct ForceRho032 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P
     static constexpr int WAVES = 6;
@@ -296,6 +299,7 @@ struct ForceRho032 {
 // staged as rho - rho0 in FP32, so their difference keeps its precision.
 struct Continuity32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     using Lds = Lds32Full;
     static constexpr bool STAGE_Q2 = false;  // pos.w = rho - rho0
     static constexpr int WAVES = 5;
@@ -368,6 +372,7 @@ struct Continuity32 {
 template <bool EMIT>
 struct Pairs32 {
     static constexpr bool INTERACT = EMIT;
+    static constexpr bool USES_Q = true;  // interact reads the candidate index
     using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;
     static constexpr int WAVES = 5;
@@ -423,6 +428,7 @@ struct Pairs32 {
 
 struct Digest32 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = true;  // interact reads the candidate index
     using Lds = Lds32Pos;
     static constexpr bool STAGE_Q2 = true;
     static constexpr int WAVES = 5;
@@ -475,6 +481,14 @@ struct Digest32 {
 };
 
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
+// Entries of the candidate-index ring: the kept-list kernel of an operator
+// that does not read the index keeps none (LDS for one more wave per SIMD).
+template <class Op, bool KEPT>
+constexpr int cands_q()
+{
+    return KEPT && !Op::USES_Q ? 1 : 128;
+}
+
 template <int V, bool FLAT, int T, class Op>
 __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
                               const typename Op::Lds& lds, const int* q, unsigned long long w,
@@ -490,14 +504,16 @@ __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename
             const float dx = ht.x - a.x, dy = ht.y - a.y;
             const float dz = FLAT ? 0.0f : ht.z - a.z;
             const float ss = FLAT ? fmaf(dy, dy, dx * dx) : fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            Op::template interact<V>(c, d, ht, lds, jj, hp, q[jj], dx, dy, dz, ss, part);
+            Op::template interact<V>(c, d, ht, lds, jj, hp, Op::USES_Q ? q[jj] : 0, dx, dy, dz, ss,
+                                     part);
         }
     }
 }
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op, bool KEPT>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, bool kept_ok, typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
+                              bool store_masks, bool kept_ok, typename Op::Lds& lds,
+                              Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()>& L,
                               Bal<typename Op::Part>& bal, unsigned& npairs,
                               unsigned long long& ntests)
 {
@@ -569,7 +585,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
                 Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & 127, d, ox,
                                        oy, oz, lds);
-                L.q[k & 127] = q;
+                if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & 127] = q;
             }
             fill = min(nk, head + 64);
             last = true;
@@ -601,7 +617,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 const int k = head + lane;
                 if (k >= fill) {
                     Op::template put<FLAT>(pad_raw(), k & 127, d, ox, oy, oz, lds);
-                    L.q[k & 127] = first;
+                    if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & 127] = first;
                     if constexpr (PERIODIC) L.rg[k & 127] = 0;
                 }
                 __syncthreads();
@@ -727,7 +743,7 @@ __global__ __launch_bounds__(64, KEPT ? Op::WAVES_KEPT : Op::WAVES) void k_pair3
                                                                                int out_col)
 {
     __shared__ typename Op::Lds lds;
-    __shared__ Cands<ROWS, PERIODIC> L;
+    __shared__ Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()> L;
     __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;

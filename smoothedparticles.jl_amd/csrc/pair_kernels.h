@@ -174,23 +174,24 @@ __device__ inline int lowest_bit(MaskT<T> m)
 // The kept candidates are staged in a ring of 128 slots: q (and rg, for
 // periodic systems) of each slot, and the kept position of the first
 // target at each occurrence of the tile's cell.
-template <int ROWS, bool PER>
+template <int ROWS, bool PER, int NQ = 128>
 struct Cands {
     static constexpr int MAXR = 2 * ROWS;
     static constexpr int NSH = PER ? MAXR : 1;
     int j0[MAXR];
     int pref[MAXR + 1];
     double sx[NSH], sy[NSH], sz[NSH];  // image shift (periodic systems)
-    int q[128];                        // particle index of each ring slot
+    int q[NQ];                         // particle index of each ring slot (1: not kept,
+                                       // in kernels whose operator does not need it)
     int rg[PER ? 128 : 1];             // range of each ring slot
     int self0[MAXR];
 };
 
 // Uniform work (every lane computes the same values); lane 0 writes.
 // Returns the number of ranges; *total receives the candidate count.
-template <int ROWS, bool PERIODIC>
+template <int ROWS, bool PERIODIC, int NQ>
 __device__ inline int build_cands(const SysDesc& d, const int* cell_start, int lc,
-                                  Cands<ROWS, PERIODIC>& L, int& total)
+                                  Cands<ROWS, PERIODIC, NQ>& L, int& total)
 {
     const bool w = __lane_id() == 0;
     int n = 0, tot = 0;
@@ -220,8 +221,8 @@ __device__ inline int build_cands(const SysDesc& d, const int* cell_start, int l
 
 // Slot of chunk `base` that holds candidate g (g < total): its range and
 // particle index.  Ranges are scanned from `cur`, the range of slot 0.
-template <int ROWS, bool PER>
-__device__ inline int slot_range(const Cands<ROWS, PER>& L, int cur, int g)
+template <int ROWS, bool PER, int NQ>
+__device__ inline int slot_range(const Cands<ROWS, PER, NQ>& L, int cur, int g)
 {
     int i = cur;
     while (L.pref[i + 1] <= g) ++i;
@@ -229,15 +230,15 @@ __device__ inline int slot_range(const Cands<ROWS, PER>& L, int cur, int g)
 }
 
 // Range of the next chunk's slot 0.
-template <int ROWS, bool PER>
-__device__ inline int advance_range(const Cands<ROWS, PER>& L, int nr, int cur, int next_base)
+template <int ROWS, bool PER, int NQ>
+__device__ inline int advance_range(const Cands<ROWS, PER, NQ>& L, int nr, int cur, int next_base)
 {
     while (cur < nr && L.pref[cur + 1] <= next_base) ++cur;
     return cur;
 }
 
-template <int ROWS>
-__device__ inline Seg range_shift(const Cands<ROWS, true>& L, int i)
+template <int ROWS, int NQ>
+__device__ inline Seg range_shift(const Cands<ROWS, true, NQ>& L, int i)
 {
     Seg g;
     g.j0 = g.j1 = 0;
@@ -258,16 +259,16 @@ __device__ inline int lane_rank(unsigned long long m)
 // far; nself: occurrences of the first target so far.  Returns the lane's
 // kept position (its ring slot is pos & 127), -1 if its candidate is not
 // kept.
-template <int ROWS, bool PER>
-__device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool keep, int q, int rg,
-                              int first)
+template <int ROWS, bool PER, int NQ>
+__device__ inline int compact(Cands<ROWS, PER, NQ>& L, int& fill, int& nself, bool keep, int q,
+                              int rg, int first)
 {
     const unsigned long long km = __ballot(keep);
     const unsigned long long sm = __ballot(keep && q == first);
     const int pos = fill + lane_rank(km);
     if (keep) {
         const int r = pos & 127;
-        L.q[r] = q;
+        if constexpr (NQ > 1) L.q[r] = q;
         if constexpr (PER) L.rg[r] = rg;
         if (q == first) L.self0[nself + lane_rank(sm)] = pos;
     }
@@ -280,8 +281,8 @@ __device__ inline int compact(Cands<ROWS, PER>& L, int& fill, int& nself, bool k
 // that hold the target itself (the reference skips p === q,
 // src/core.jl:105): target t sits t slots after the first target at every
 // occurrence of the tile's cell.
-template <int ROWS, bool PER, int T>
-__device__ inline void drop_self(const Cands<ROWS, PER>& L, int nself, int head, int t, int s,
+template <int ROWS, bool PER, int T, int NQ>
+__device__ inline void drop_self(const Cands<ROWS, PER, NQ>& L, int nself, int head, int t, int s,
                                  MaskT<T>& m)
 {
     constexpr int S = 64 / T;
@@ -520,8 +521,8 @@ __device__ inline void fetch_pos(const PairCtx& c, int q, const Seg& g, Raw& r)
 
 // Fetch candidate g of the list (a pad past its end) into registers; cur is
 // the range of the chunk's first slot.  rq, rg: its particle index and range.
-template <int ROWS, bool PERIODIC, class Op>
-__device__ inline void fetch_slot(const PairCtx& c, const Cands<ROWS, PERIODIC>& L, int total,
+template <int ROWS, bool PERIODIC, class Op, int NQ>
+__device__ inline void fetch_slot(const PairCtx& c, const Cands<ROWS, PERIODIC, NQ>& L, int total,
                                   int cur, int g, Raw& raw, int& rq, int& rg)
 {
     if (g < total) {
