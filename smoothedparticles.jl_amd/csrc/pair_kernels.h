@@ -333,8 +333,9 @@ __device__ inline float box_dist2(const Box<float>& b, float x, float y, float z
 //   - the owner lane of target t (slice 0, lane t) publishes the target's
 //     64-bit word of accepted slots in LDS;
 //   - the pairs are cut into runs of at most B pairs, one target per run,
-//     with B the smallest quota (from the mean up) for which the runs fit
-//     in 64 lanes; lane l takes run l;
+//     with B a quota from the mean up for which the runs fit in 64 lanes
+//     (or, when every target is one run, each owner lane walks its own);
+//     lane l takes run l;
 //   - a lane reads its target's data from the owner lane (shuffles), skips
 //     to its run with a popcount descent and walks the run;
 //   - its partial sums go to LDS, and the owner folds the runs of its target
@@ -455,21 +456,30 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
         own = true;
         return tot;
     }
-    // The run length B is the smallest with at most 64 runs.  The number of
-    // runs does not grow with B, is at most 64 at B = max(pop) (one run per
-    // lane with pairs) and cannot be at most 64 below ceil(tot / 64).
-    const int maxp = wave_max(pop);
+    // The run length B: the mean (rounded up) or one more if the runs then
+    // fit in 64 lanes; else at most two bisection steps between that and
+    // ceil(tot / (64 - nz)) (nz: lanes with pairs), which always fits since
+    // ceil(pop / B) <= pop / B + 1.  The number of runs does not grow with
+    // B; at B >= max(pop) every target is one run.  A full search for the
+    // smallest B costs more tests than it saves pairs on sparse chunks.
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
-    if (B < maxp && wave_sum7(runs_of(pop, B, run_magic(B))) > 64) {
-        int lo = B + 1, hi = maxp;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (wave_sum7(runs_of(pop, mid, run_magic(mid))) > 64)
-                lo = mid + 1;
-            else
-                hi = mid;
+    int need = wave_sum7(runs_of(pop, B, run_magic(B)));
+    const int maxp = wave_max(pop);
+    if (need > 64 && B < maxp) {
+        ++B;
+        need = wave_sum7(runs_of(pop, B, run_magic(B)));
+        if (need > 64 && B < maxp) {
+            const int nz = __popcll(__ballot(pop > 0));  // < 64: not every lane has pairs
+            int lo = B + 1, hi = min((tot + (64 - nz) - 1) / (64 - nz), maxp);
+            for (int it = 0; it < 2 && lo < hi; ++it) {
+                const int mid = (lo + hi) >> 1;
+                if (wave_sum7(runs_of(pop, mid, run_magic(mid))) > 64)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            B = hi;
         }
-        B = lo;
     }
     if (B >= maxp) {
         own = true;
