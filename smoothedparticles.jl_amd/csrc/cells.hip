@@ -399,8 +399,9 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
                                                long long kept_cap, int4* tiles, int* meta,
                                                unsigned long long* mask_total)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c == 0) {
+    __shared__ int with_tiles;
+    if (threadIdx.x == 0) with_tiles = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         meta[META_NTILES] = tile_start[ncells];
         const unsigned long long need = mask_start[ncells];
         *mask_total = need;
@@ -408,24 +409,33 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
         // 64 kept-candidate entries per chunk of a tile, T mask words
         meta[META_KEPT_OK] = (long long)(need / T * 64) <= kept_cap ? 1 : 0;
     }
-    const int t0 = c < ncells ? tile_start[c] : 0;
-    const int nt = c < ncells ? tile_start[c + 1] - t0 : 0;
-    // cells with tiles, one atomic per wave
-    const unsigned long long wt = __ballot(nt > 0);
-    if (__lane_id() == 0 && wt) atomicAdd(&meta[META_NCELLS], __popcll(wt));
-    if (c >= ncells) return;
-    const int cs = cell_start[c];
-    const long long words = (long long)chunks[c] * T;
-    for (int t = 0; t < nt; ++t) {
-        const unsigned long long base = mask_start[c] + (unsigned long long)(t * words);
-        tiles[t0 + t] = make_int4(c, cs + t * T, (int)(unsigned)(base & 0xffffffffull),
-                                  (int)(base >> 32));
+    __syncthreads();
+    // cells with tiles: counted per block, one global atomic per block (a
+    // single address serialises its atomics)
+    int n = 0;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) {
+        const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
+        n += nt > 0;
+        const int cs = cell_start[c];
+        const long long words = (long long)chunks[c] * T;
+        for (int t = 0; t < nt; ++t) {
+            const unsigned long long base = mask_start[c] + (unsigned long long)(t * words);
+            tiles[t0 + t] = make_int4(c, cs + t * T, (int)(unsigned)(base & 0xffffffffull),
+                                      (int)(base >> 32));
+        }
     }
+    if (n) atomicAdd(&with_tiles, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && with_tiles) atomicAdd(&meta[META_NCELLS], with_tiles);
 }
 
 // Bounding box of each tile's targets (absolute FP64), read by the pair
 // kernels to cull candidates.  A wave covers 64 / T tiles at a time, one
-// lane per target; NaN coordinates are ignored (fmin/fmax).
+// lane per target.  The coordinates go through LDS; of the T lanes of a
+// tile, G = T / 8 lanes per bound (c < 6: min x, max x, min y, ...) reduce
+// eight coordinates each and combine in log2(G) shuffle steps.  Targets
+// past the cell's end take the first target's place; NaN coordinates are
+// ignored (fmin/fmax).
 template <bool FLAT>
 __global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ tiles,
                                                     const int* __restrict__ cell_start,
@@ -434,74 +444,42 @@ __global__ __launch_bounds__(256) void k_tile_boxes(const int4* __restrict__ til
                                                     const double* __restrict__ z, int T,
                                                     double* boxes)
 {
+    __shared__ double xs[4][3][64];
     const int ntiles = meta[META_NTILES];
     const int lane = threadIdx.x & 63;
-    const int t = lane & (T - 1);
-    const long long nwaves = (long long)gridDim.x * 4;
-    if (T == 64) {
-        // one tile per wave: the coordinates go through LDS, lane 8 c + j
-        // (c < 6: min x, max x, min y, ...) reduces eight of them and the
-        // eight lanes of c combine theirs in three shuffle steps
-        __shared__ double xs[4][3][64];
-        double(*w)[64] = xs[threadIdx.x >> 6];
-        const int c = lane >> 3, j = lane & 7, axis = c >> 1;
-        const bool lo = (c & 1) == 0;
-        // the block's four waves step together (block barriers)
-        for (long long b = (long long)blockIdx.x * 4; b < ntiles; b += nwaves) {
-            const long long i = b + (threadIdx.x >> 6);
-            if (i < ntiles) {
-                const int4 td = tiles[i];
-                const int p = td.y + lane;
-                // targets past the cell's end take the first target's place
-                const int pp = p < cell_start[td.x + 1] ? p : td.y;
-                w[0][lane] = x[pp];
-                w[1][lane] = y[pp];
-                w[2][lane] = FLAT ? 0.0 : z[pp];
-            }
-            __syncthreads();
-            double v = 0.0;
-            if (c < 6) {
-                v = w[axis][8 * j];
-                for (int k = 1; k < 8; ++k) {
-                    const double u = w[axis][8 * j + k];
-                    v = lo ? fmin(v, u) : fmax(v, u);
-                }
-            }
-            for (int o = 1; o < 8; o <<= 1) {
-                const double u = __shfl_xor(v, o);
-                v = lo ? fmin(v, u) : fmax(v, u);
-            }
-            if (i < ntiles && c < 6 && j == 0) boxes[6 * i + c] = v;
-            __syncthreads();
-        }
-        return;
-    }
-    for (long long wave = ((long long)blockIdx.x * 256 + threadIdx.x) >> 6;
-         wave * (64 / T) < ntiles; wave += nwaves) {
-        const long long i = wave * (64 / T) + lane / T;
-        double b[6] = {1.0e30, -1.0e30, 1.0e30, -1.0e30, 0.0, 0.0};
-        if (!FLAT) b[4] = 1.0e30, b[5] = -1.0e30;
+    double(*w)[64] = xs[threadIdx.x >> 6];
+    const int per_wave = 64 / T, G = T / 8;
+    const int tw = lane / T, lt = lane % T;  // tile in the wave, target in the tile
+    const int c = lt / G, j = lt % G, axis = c >> 1;
+    const bool lo = (c & 1) == 0;
+    const long long per_block = 4ll * per_wave, step = (long long)gridDim.x * per_block;
+    // the block's four waves step together (block barriers)
+    for (long long b = (long long)blockIdx.x * per_block; b < ntiles; b += step) {
+        const long long i = b + (threadIdx.x >> 6) * per_wave + tw;
         if (i < ntiles) {
             const int4 td = tiles[i];
-            const int p = td.y + t;
-            if (p < cell_start[td.x + 1]) {
-                b[0] = b[1] = x[p];
-                b[2] = b[3] = y[p];
-                if (!FLAT) b[4] = b[5] = z[p];
+            const int p = td.y + lt;
+            const int pp = p < cell_start[td.x + 1] ? p : td.y;
+            w[0][lane] = x[pp];
+            w[1][lane] = y[pp];
+            w[2][lane] = FLAT ? 0.0 : z[pp];
+        }
+        __syncthreads();
+        double v = 0.0;
+        if (c < 6) {
+            const int k0 = tw * T + 8 * j;
+            v = w[axis][k0];
+            for (int k = 1; k < 8; ++k) {
+                const double u = w[axis][k0 + k];
+                v = lo ? fmin(v, u) : fmax(v, u);
             }
         }
-        for (int o = 1; o < T; o <<= 1) {
-            b[0] = fmin(b[0], __shfl_xor(b[0], o)); b[1] = fmax(b[1], __shfl_xor(b[1], o));
-            b[2] = fmin(b[2], __shfl_xor(b[2], o)); b[3] = fmax(b[3], __shfl_xor(b[3], o));
-            if (!FLAT) {
-                b[4] = fmin(b[4], __shfl_xor(b[4], o)); b[5] = fmax(b[5], __shfl_xor(b[5], o));
-            }
+        for (int o = 1; o < G; o <<= 1) {
+            const double u = __shfl_xor(v, o);
+            v = lo ? fmin(v, u) : fmax(v, u);
         }
-        if (i < ntiles && t < 6) {
-            double v = b[0];
-            for (int k = 1; k < 6; ++k) v = t == k ? b[k] : v;
-            boxes[6 * i + t] = v;
-        }
+        if (i < ntiles && c < 6 && j == 0) boxes[6 * i + c] = v;
+        __syncthreads();
     }
 }
 
@@ -521,7 +499,7 @@ void enqueue_tiles(sph_system* s)
                                                      s->cell_chunks);
     scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, ncells, s->tile_T}, ncells + 1,
          s->mask_start, s->scan_part64);
-    k_tiles<<<nbc, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
+    k_tiles<<<nbc < 256 ? nbc : 256, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
                                         s->mask_start, ncells, s->tile_T,
                                         s->masks ? s->mask_cap : 0,
                                         s->kept && (s->kept_rg || !s->periodic) ? s->kept_cap : 0,
