@@ -261,7 +261,7 @@ def serve(count=48, rate=2.5, seed=0, budget_gb=32.0):
         "value": srv.pairs / srv.wall, "unit": "pair-interactions/s",
         "wall_s": srv.wall, "p50_s": lat["p50_s"], "p99_s": lat["p99_s"],
         "mean_latency_s": lat["mean_s"], "peak_concurrent": srv.peak_active,
-        "budget_GB": budget_gb, "peak_admitted_GB": srv.peak_in_use / 1e9,
+        "budget_GB": srv.budget / 1e9, "peak_admitted_GB": srv.peak_in_use / 1e9,
         "peak_device_footprint_GB": srv.peak_footprint / 1e9, "step_graphs": srv.graph,
     }
     srv.close()

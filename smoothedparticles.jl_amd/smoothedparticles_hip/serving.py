@@ -17,8 +17,11 @@ kernels overlap on the device.
 
 Admission is by device footprint: a request is charged its particle count
 times the bytes per particle its batch holds, measured with
-sph_batch_footprint after every admission (the largest ratio seen; a
-starting value until the first measurement).
+sph_batch_footprint after an admission that brings the batch to a new
+largest particle count (a starting value until the first measurement).  A
+batch keeps its device arrays when requests retire, so the ratio is taken
+only at that high-water mark, where it measures what the particles need
+rather than capacity left over from larger past batches.
 """
 import math
 import time
@@ -147,6 +150,7 @@ class Server:
         self.active = {2: [], 3: []}
         self.bpp = dict(BYTES_PER_PARTICLE)  # admission charge per particle, per dimension
         self.measured = {2: False, 3: False}
+        self.n_high = {2: 0, 3: 0}           # largest live particle count of each batch
         self.charge = {}                     # request id -> bytes charged
         self.in_use = 0
         self.peak_in_use = 0
@@ -175,9 +179,9 @@ class Server:
         """Bytes per live particle the batch of `dim` holds (sph_batch_footprint)."""
         b = self.batches[dim]
         _, n = b.size()
-        if n > 0:
-            ratio = b.footprint() / n
-            self.bpp[dim] = ratio if not self.measured[dim] else max(self.bpp[dim], ratio)
+        if n > 0 and n >= self.n_high[dim]:
+            self.n_high[dim] = n
+            self.bpp[dim] = b.footprint() / n
             self.measured[dim] = True
         self.peak_footprint = max(self.peak_footprint,
                                   sum(x.footprint() for x in self.batches.values()))
