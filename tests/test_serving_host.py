@@ -27,3 +27,42 @@ def test_state_size_follows_request(kind, n):
     r = serving.Request(0, kind, n, 100, 0.0)
     assert r.footprint() == serving.BYTES_PER_PARTICLE[r.dim] * len(st.x)
     assert r.footprint(1500.5) == int(np.ceil(1500.5 * len(st.x)))
+
+
+class _FakeBatch:
+    """Stands in for a device batch: a live particle count and the bytes
+    the library would report for it (capacity kept when particles leave)."""
+
+    def __init__(self):
+        self.n = 0
+        self.bytes = 0
+
+    def size(self):
+        return 1, self.n
+
+    def footprint(self):
+        return self.bytes
+
+
+def test_admission_charge_ignores_capacity_kept_from_retired_requests():
+    """The per-particle charge is measured only when the batch reaches a new
+    largest particle count: after a large request retires, the batch keeps
+    its arrays, and footprint / n would otherwise charge every later request
+    as if it needed the retired one's memory."""
+    srv = serving.Server.__new__(serving.Server)
+    srv.bpp = dict(serving.BYTES_PER_PARTICLE)
+    srv.measured = {2: False, 3: False}
+    srv.n_high = {2: 0, 3: 0}
+    srv.peak_footprint = 0
+    b = _FakeBatch()
+    srv.batches = {3: b}
+    b.n, b.bytes = 400_000, 400_000 * 900      # a large request admitted
+    srv._measure(3)
+    assert srv.measured[3] and srv.bpp[3] == 900
+    b.n = 2_000                                # it retired; its memory stays
+    srv._measure(3)
+    assert srv.bpp[3] == 900                   # not 180,000 B per particle
+    b.n, b.bytes = 500_000, 500_000 * 850      # a new high-water mark
+    srv._measure(3)
+    assert srv.bpp[3] == 850
+    assert srv.peak_footprint == 500_000 * 850
