@@ -19,12 +19,14 @@ __device__ inline float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // LDS ring of kept candidates (FP32): (x, y, z, |x|^2) -- (x, y, z,
 // P/rho^2) for the force pass -- and, for the force pass, (vx, vy, vz, rho).
+template <int N>
 struct Lds32Pos {
-    float4 pos[128];
+    float4 pos[N];
 };
 
+template <int N>
 struct Lds32Full {
-    float4 pos[128], v[128];
+    float4 pos[N], v[N];
 };
 
 // Tile-relative FP32 position of a fetched candidate and its |.|^2.
@@ -40,7 +42,8 @@ __device__ inline float4 rel_pos32(const Raw& r, double ox, double oy, double oz
 struct Density32 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = false;
-    using Lds = Lds32Pos;
+    template <int N>
+    using Lds = Lds32Pos<N>;
     static constexpr bool STAGE_Q2 = true;  // pos.w = |x|^2
     static constexpr int WAVES = 6;  // a few spilled registers outside the loops
     static constexpr int WAVES_KEPT = 6;
@@ -68,9 +71,9 @@ struct Density32 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
-                                    double oz, Lds& s)
+                                    double oz, L& s)
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
@@ -78,8 +81,8 @@ struct Density32 {
     static __device__ void zero(Part& q) { q.w = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const L&,
                                     int, int, int, float, float, float, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
@@ -105,7 +108,8 @@ struct Density32 {
 struct Force32 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = false;
-    using Lds = Lds32Full;
+    template <int N>
+    using Lds = Lds32Full<N>;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P/rho^2
     static constexpr int WAVES = 5;  // spills a few registers; LDS fits five blocks too
     static constexpr int WAVES_KEPT = 6;
@@ -148,9 +152,9 @@ struct Force32 {
         r.a4 = c.prho2[q];
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc& d, double ox, double oy,
-                                    double oz, Lds& s)
+                                    double oz, L& s)
     {
         float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
         a.w = (float)r.a4;
@@ -165,9 +169,9 @@ struct Force32 {
     {
         return (d.ph.avk != 0.0 ? 1 : 0) | (d.ph.mu != 0.0 ? 2 : 0);
     }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, float dx, float dy,
+                                    const L& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
         const float4 v = s.v[jj];
@@ -220,7 +224,8 @@ struct Force32 {
 struct ForceRho032 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = false;
-    using Lds = Lds32Pos;
+    template <int N>
+    using Lds = Lds32Pos<N>;
     static constexpr bool STAGE_Q2 = false;  // pos.w = P
     static constexpr int WAVES = 6;
     static constexpr int WAVES_KEPT = 6;
@@ -250,9 +255,9 @@ struct ForceRho032 {
         r.a0 = COL(c, C_P, q);
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
-                                    double oz, Lds& s)
+                                    double oz, L& s)
     {
         float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
         a.w = (float)r.a0;
@@ -262,9 +267,9 @@ struct ForceRho032 {
     static __device__ void zero(Part& q) { q.ax = q.ay = q.az = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, float dx, float dy,
+                                    const L& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
@@ -300,7 +305,8 @@ struct ForceRho032 {
 struct Continuity32 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = false;
-    using Lds = Lds32Full;
+    template <int N>
+    using Lds = Lds32Full<N>;
     static constexpr bool STAGE_Q2 = false;  // pos.w = rho - rho0
     static constexpr int WAVES = 5;
     static constexpr int WAVES_KEPT = 5;
@@ -332,9 +338,9 @@ struct Continuity32 {
         r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc& d, double ox, double oy,
-                               double oz, Lds& s)
+                               double oz, L& s)
     {
         float4 a = rel_pos32<FLAT>(r, ox, oy, oz);
         a.w = (float)(r.a0 - d.ph.rho0);
@@ -345,9 +351,9 @@ struct Continuity32 {
     static __device__ void zero(Part& q) { q.d = 0.0f; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, float dx, float dy,
+                                    const L& s, int jj, int, int, float dx, float dy,
                                     float dz, float ss, Part& q)
     {
         const float r = sqrt_hw(ss);
@@ -373,7 +379,8 @@ template <bool EMIT>
 struct Pairs32 {
     static constexpr bool INTERACT = EMIT;
     static constexpr bool USES_Q = true;  // interact reads the candidate index
-    using Lds = Lds32Pos;
+    template <int N>
+    using Lds = Lds32Pos<N>;
     static constexpr bool STAGE_Q2 = true;
     static constexpr int WAVES = 5;
     static constexpr int WAVES_KEPT = 5;
@@ -401,9 +408,9 @@ struct Pairs32 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
-                                    double oz, Lds& s)
+                                    double oz, L& s)
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
@@ -411,8 +418,8 @@ struct Pairs32 {
     static __device__ void zero(Part&) {}
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const L&,
                                     int, int p, int q, float, float, float, float, Part&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
@@ -429,7 +436,8 @@ struct Pairs32 {
 struct Digest32 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = true;  // interact reads the candidate index
-    using Lds = Lds32Pos;
+    template <int N>
+    using Lds = Lds32Pos<N>;
     static constexpr bool STAGE_Q2 = true;
     static constexpr int WAVES = 5;
     static constexpr int WAVES_KEPT = 5;
@@ -455,9 +463,9 @@ struct Digest32 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    template <bool FLAT>
+    template <bool FLAT, class L>
     static __device__ void put(const Raw& r, int l, const SysDesc&, double ox, double oy,
-                                    double oz, Lds& s)
+                                    double oz, L& s)
     {
         s.pos[l] = rel_pos32<FLAT>(r, ox, oy, oz);
     }
@@ -465,8 +473,8 @@ struct Digest32 {
     static __device__ void zero(Part& a) { a.h = a.n = 0; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const L&,
                                     int, int, int q, float, float, float, float, Part& a)
     {
         a.h += mix64((unsigned long long)c.refidx[q] + 1ull);
@@ -481,6 +489,16 @@ struct Digest32 {
 };
 
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
+// Slots of the LDS ring of staged candidates.  A testing pass appends the
+// kept candidates of each streamed step behind those not yet worked, so it
+// needs two chunks' room; the kept-list kernel stages one chunk per step and
+// works it before the next, so one chunk's room does (less LDS, more waves).
+template <bool KEPT>
+constexpr int ring_slots()
+{
+    return KEPT ? 64 : 128;
+}
+
 // Entries of the candidate-index ring: the kept-list kernel of an operator
 // that does not read the index keeps none (LDS for one more wave per SIMD).
 template <class Op, bool KEPT>
@@ -489,9 +507,9 @@ constexpr int cands_q()
     return KEPT && !Op::USES_Q ? 1 : 128;
 }
 
-template <int V, bool FLAT, int T, class Op>
+template <int V, bool FLAT, int T, class Op, class L>
 __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
-                              const typename Op::Lds& lds, const int* q, unsigned long long w,
+                              const L& lds, const int* q, unsigned long long w,
                               int cnt, int rb, int hp, typename Op::Part& part)
 {
     if constexpr (V < Op::NVAR) {
@@ -512,7 +530,8 @@ __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename
 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op, bool KEPT>
 __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load_masks,
-                              bool store_masks, bool kept_ok, typename Op::Lds& lds,
+                              bool store_masks, bool kept_ok,
+                              typename Op::template Lds<ring_slots<KEPT>()>& lds,
                               Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()>& L,
                               Bal<typename Op::Part>& bal, unsigned& npairs,
                               unsigned long long& ntests)
@@ -550,6 +569,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
     // testing pass wrote; otherwise it streams the candidate list and culls
     // it again, keeping everything within the guard band of the box
     constexpr bool use_kept = KEPT;  // (the kernel took the tiles whose masks and lists are loaded)
+    constexpr int RS = ring_slots<KEPT>();
     const bool store_kept = !KEPT && store_masks && kept_ok;
     const long long lbase = mbase / T * 64;
     const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
@@ -583,9 +603,9 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 const int q = c.kept[lbase + k];
                 Seg sh{};
                 if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
-                Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & 127, d, ox,
+                Op::template put<FLAT>(Op::template fetch<PERIODIC>(c, q, sh), k & (RS - 1), d, ox,
                                        oy, oz, lds);
-                if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & 127] = q;
+                if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & (RS - 1)] = q;
             }
             fill = min(nk, head + 64);
             last = true;
@@ -602,7 +622,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                                (unsigned)(rq - first) < (unsigned)nvalid);
             const int pos = compact(L, fill, nself, keep, rq, rrg, first);
             if (pos >= 0) {
-                Op::template put<FLAT>(raw, pos & 127, d, ox, oy, oz, lds);
+                Op::template put<FLAT>(raw, pos & (RS - 1), d, ox, oy, oz, lds);
                 if (store_kept) {
                     c.kept[lbase + pos] = rq;
                     if constexpr (PERIODIC) c.kept_rg[lbase + pos] = (unsigned char)rrg;
@@ -616,13 +636,13 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 // the last chunk: pad its free slots
                 const int k = head + lane;
                 if (k >= fill) {
-                    Op::template put<FLAT>(pad_raw(), k & 127, d, ox, oy, oz, lds);
-                    if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & 127] = first;
-                    if constexpr (PERIODIC) L.rg[k & 127] = 0;
+                    Op::template put<FLAT>(pad_raw(), k & (RS - 1), d, ox, oy, oz, lds);
+                    if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & (RS - 1)] = first;
+                    if constexpr (PERIODIC) L.rg[k & (RS - 1)] = 0;
                 }
                 __syncthreads();
             }
-            const int rb = head & 64;  // ring half of the chunk
+            const int rb = head & (RS - 64);  // the chunk's part of the ring
             // accepted-slot word of target t (owner lanes publish it)
             unsigned long long word = 0;
             if (use_kept) {
@@ -742,7 +762,7 @@ template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op, bool KEPT>
 __global__ __launch_bounds__(64, KEPT ? Op::WAVES_KEPT : Op::WAVES) void k_pair32(PairCtx c,
                                                                                int out_col)
 {
-    __shared__ typename Op::Lds lds;
+    __shared__ typename Op::template Lds<ring_slots<KEPT>()> lds;
     __shared__ Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()> L;
     __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
