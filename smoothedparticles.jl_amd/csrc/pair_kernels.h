@@ -25,7 +25,8 @@
 // box and the cull test are the same in every pass, so all passes see the
 // same chunks.  The testing pass also records each tile's kept candidates
 // (and their ranges, for the image shifts of periodic systems); a pass that
-// loads its masks reads that list instead of streaming and culling again.
+// loads its masks reads that list instead of streaming and culling again,
+// one chunk per step, so its ring holds a single chunk (64 slots).
 //
 // Each chunk is handled in two phases:
 //   1. mask: lane (t, s) tests the chunk slots s + S*j (j < T) against
@@ -183,8 +184,8 @@ __device__ inline int lowest_bit(MaskT<T> m)
 // The candidate ranges of a tile: its cell's stencil rows, concatenated.
 // pref[i] candidates precede range i; a chunk's slot l holds candidate
 // base + l, whose range i satisfies pref[i] <= base + l < pref[i + 1].
-// The kept candidates are staged in a ring of 128 slots: q (and rg, for
-// periodic systems) of each slot, and the kept position of the first
+// The kept candidates are staged in a ring of at most 128 slots: q (and rg,
+// for periodic systems) of each slot, and the kept position of the first
 // target at each occurrence of the tile's cell.
 template <int ROWS, bool PER, int NQ = 128>
 struct Cands {
