@@ -41,17 +41,25 @@ def rel(a, b):
     return np.max(np.abs(a - b)) / np.max(np.abs(b))
 
 
-def test_fused_step_equals_operator_sequence():
+@pytest.mark.parametrize("scheme", ["wcsph_se", "wcsph_se_rho0"])
+def test_fused_step_equals_operator_sequence(scheme):
+    """The fused step (its density pass applies the EOS where it writes rho)
+    gives the bits of the operators applied one by one, for the constant-rho0
+    and the rho0-field equation of state."""
     st = states.dam_break(2, 0.02)
     p = st.params()
     g1, _ = load(st, "fp64")
     g2, _ = load(st, "fp64")
-    sp.step(g1, 5, **p)
+    eos = sp.find_pressure if scheme == "wcsph_se" else sp.find_pressure_rho0
+    if scheme == "wcsph_se_rho0":
+        for g in (g1, g2):
+            sp.init_reference_density(g, **p)
+    sp.step(g1, 5, scheme=scheme, **p)
     for _ in range(5):
         sp.create_cell_list(g2)
         sp.apply(g2, sp.reset_rho)
         sp.apply(g2, sp.find_rho, self=True, **p)
-        sp.apply(g2, sp.find_pressure, **p)
+        sp.apply(g2, eos, **p)
         sp.apply(g2, sp.reset_a)
         sp.apply(g2, sp.internal_force, **p)
         sp.apply(g2, sp.accelerate, **p)
