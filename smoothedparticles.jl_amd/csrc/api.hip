@@ -487,11 +487,9 @@ int pair_grid(long long ntiles)
 
 int run_pair(sph_system* s, int op, int self, int overwrite, unsigned long long* counter,
              long long ntiles, int prof_kernel, int mask_mode, int only_flagged = 0,
-             unsigned long long* tests = nullptr, int eos = 0)
+             unsigned long long* tests = nullptr)
 {
     PairCtx c = make_ctx(s);
-    c.eos = eos;
-    c.prho2_out = eos ? s->prho2 : nullptr;
     c.self = self;
     c.overwrite = overwrite;
     c.pair_counter = counter;
@@ -819,8 +817,7 @@ int cd_step(sph_system* s, long long tiles_bound)
 }
 
 // One step of a density-summation scheme: build; density pass (stores the
-// masks, and applies the EOS with P/rho^2 where it writes rho); force pass
-// (loads the masks); kick and drift.
+// masks); EOS with P/rho^2; force pass (loads them); kick and drift.
 int ds_step(sph_system* s, int eos, long long tiles_bound)
 {
     int rc;
@@ -828,10 +825,12 @@ int ds_step(sph_system* s, int eos, long long tiles_bound)
     prof_begin(s, ev);
     enqueue_build(s);
     prof_end(s, SPH_KERNEL_BUILD, ev);
-    // the EOS (P and P/rho^2) is applied by the density pass where it writes rho
     if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
-                       MASK_TEST_STORE, 0, s->pair_counter + 2, eos)))
+                       MASK_TEST_STORE, 0, s->pair_counter + 2)))
         return rc;
+    prof_begin(s, ev);
+    enqueue_unary(s, eos, s->prho2);
+    prof_end(s, SPH_KERNEL_UNARY, ev);
     if ((rc = run_pair(s, P_FORCE, 0, 1, s->pair_counter + 1, tiles_bound, SPH_KERNEL_FORCE,
                        MASK_LOAD)))
         return rc;

@@ -102,7 +102,6 @@ struct Density32 {
         v = v + d.ph.kW * w;
         if (c.self) v = v + d.ph.kW;
         COL(c, out_col, p) = v;
-        if (c.eos && out_col == C_RHO) eos_after_density(c, d, p, v);
     }
 };
 

@@ -74,7 +74,6 @@ struct Density64 {
         v = v + d.ph.kW * a.w;
         if (c.self) v = v + d.ph.kW;
         COL(c, OUT, p) = v;
-        if (OUT == C_RHO && c.eos) eos_after_density(c, d, p, v);
     }
 };
 

@@ -87,18 +87,6 @@ __device__ inline void count_tests(const PairCtx& c, unsigned long long n)
     if (c.test_counter && __lane_id() == 0 && n) atomicAdd(c.test_counter, n);
 }
 
-// The equation of state of the density-summation step where the density
-// pass writes rho (find_pressure!, examples/collapse_dry.jl:126, or its rho0
-// form, tests/test_collision_2d.jl:71-73): the unary operator's expressions,
-// so P and P/rho^2 have its bits, without a pass of their own.
-__device__ inline void eos_after_density(const PairCtx& c, const SysDesc& d, int p, double rho)
-{
-    const double r0 = c.eos == SPH_OP_FIND_PRESSURE_RHO0 ? COL(c, C_RHO0, p) : d.ph.rho0;
-    const double P = (d.ph.c * d.ph.c) * (rho - r0);
-    COL(c, C_P, p) = P;
-    if (c.prho2_out) c.prho2_out[p] = P / (rho * rho);
-}
-
 // Tile queues.  The tile range [0, ntiles) is cut into NQUEUE contiguous
 // parts, one per queue head.  A wave starts on queue blockIdx % NQUEUE (the
 // XCD it runs on, so neighbouring tiles share that XCD's L2), takes qgrab

@@ -123,9 +123,6 @@ struct PairCtx {
     int self;                 // add the self term (density)
     int overwrite;            // write instead of accumulate
     int only_flagged;         // skip systems whose pass_on is 0
-    int eos;                  // density pass: 0, or the EOS operator (SPH_OP_FIND_PRESSURE,
-                              // SPH_OP_FIND_PRESSURE_RHO0) applied where rho is written
-    double* prho2_out;        // P/rho^2 written by that EOS
 };
 
 // Tile queues of the pair kernels: the tiles are split into NQUEUE
