@@ -43,9 +43,8 @@ def rel(a, b):
 
 @pytest.mark.parametrize("scheme", ["wcsph_se", "wcsph_se_rho0"])
 def test_fused_step_equals_operator_sequence(scheme):
-    """The fused step (its density pass applies the EOS where it writes rho)
-    gives the bits of the operators applied one by one, for the constant-rho0
-    and the rho0-field equation of state."""
+    """The fused step gives the bits of the operators applied one by one,
+    for the constant-rho0 and the rho0-field equation of state."""
     st = states.dam_break(2, 0.02)
     p = st.params()
     g1, _ = load(st, "fp64")
