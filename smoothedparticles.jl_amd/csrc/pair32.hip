@@ -489,24 +489,6 @@ struct Digest32 {
 };
 
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
-// Slots of the LDS ring of staged candidates.  A testing pass appends the
-// kept candidates of each streamed step behind those not yet worked, so it
-// needs two chunks' room; the kept-list kernel stages one chunk per step and
-// works it before the next, so one chunk's room does (less LDS, more waves).
-template <bool KEPT>
-constexpr int ring_slots()
-{
-    return KEPT ? 64 : 128;
-}
-
-// Entries of the candidate-index ring: the kept-list kernel of an operator
-// that does not read the index keeps none (LDS for one more wave per SIMD).
-template <class Op, bool KEPT>
-constexpr int cands_q()
-{
-    return KEPT && !Op::USES_Q ? 1 : 128;
-}
-
 template <int V, bool FLAT, int T, class Op, class L>
 __device__ inline void walk32(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
                               const L& lds, const int* q, unsigned long long w,

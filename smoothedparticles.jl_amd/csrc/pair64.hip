@@ -7,25 +7,37 @@ namespace sph {
 
 // LDS ring of kept candidates (FP64): positions only, or xy, (z, rho),
 // (vx, vy), (vz, aux) where aux is P/rho^2 (force) or P (force with rho0).
+template <int N>
 struct Lds64Pos {
-    double2 xy[128];
-    double z[128];
+    double2 xy[N];
+    double z[N];
 };
 
+template <int N>
 struct Lds64Full {
-    double2 xy[128], zr[128], v01[128], v2a[128];
+    double2 xy[N], zr[N], v01[N], v2a[N];
 };
 
-__device__ inline double lds_z(const Lds64Pos& s, int j) { return s.z[j]; }
-__device__ inline double lds_z(const Lds64Full& s, int j) { return s.zr[j].x; }
+template <int N>
+__device__ inline double lds_z(const Lds64Pos<N>& s, int j)
+{
+    return s.z[j];
+}
+template <int N>
+__device__ inline double lds_z(const Lds64Full<N>& s, int j)
+{
+    return s.zr[j].x;
+}
 
 // rho += m W (OUT = C_RHO) or rho0 += m W (OUT = C_RHO0).
 template <int OUT>
 struct Density64 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     static constexpr int WAVES = 5;  // occupancy the registers are fitted to
     static constexpr int WAVES_KEPT = 5;
-    using Lds = Lds64Pos;
+    template <int N>
+    using Lds = Lds64Pos<N>;
     struct Tgt {
         double x, y, z;
     };
@@ -44,7 +56,8 @@ struct Density64 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.z[l] = r.z;
@@ -55,8 +68,8 @@ struct Density64 {
     static __device__ void fold(Acc& a, const Part& q) { a.w += q.w; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt&, const L&,
                                     int, int, int, double, double, double, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -84,9 +97,11 @@ struct Density64 {
 //   laminar   a += 2 kdW mu / rho0^2 u^3 v_pq
 struct Force64 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     static constexpr int WAVES = 3;  // occupancy the registers are fitted to
-    static constexpr int WAVES_KEPT = 3;
-    using Lds = Lds64Full;
+    static constexpr int WAVES_KEPT = 4;  // 125 VGPRs; LDS 7.9 KB with the one-chunk ring
+    template <int N>
+    using Lds = Lds64Full<N>;
     struct Tgt {
         double x, y, z, vx, vy, vz, pr, rho;
         bool fluid;
@@ -113,7 +128,8 @@ struct Force64 {
         r.a4 = c.prho2[q];
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.zr[l] = make_double2(r.z, r.a0);
@@ -133,9 +149,9 @@ struct Force64 {
     {
         return (d.ph.avk != 0.0 ? 1 : 0) | (d.ph.mu != 0.0 ? 2 : 0);
     }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, double dx, double dy,
+                                    const L& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -179,9 +195,11 @@ struct Force64 {
 // internal_force! of tests/test_collision_2d.jl:75-78 (P/rho0^2, all targets).
 struct ForceRho064 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     static constexpr int WAVES = 4;  // occupancy the registers are fitted to
     static constexpr int WAVES_KEPT = 4;
-    using Lds = Lds64Full;
+    template <int N>
+    using Lds = Lds64Full<N>;
     struct Tgt {
         double x, y, z, P;
     };
@@ -201,7 +219,8 @@ struct ForceRho064 {
         r.a0 = COL(c, C_P, q);
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.zr[l].x = r.z;
@@ -213,9 +232,9 @@ struct ForceRho064 {
     static __device__ void fold(Acc& a, const Part& q) { a.ax += q.ax; a.ay += q.ay; a.az += q.az; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, double dx, double dy,
+                                    const L& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -244,9 +263,11 @@ struct ForceRho064 {
 //   Drho += kdW u^3 (x_pq.v_pq + 2 nu (rho_p - rho_q))
 struct Continuity64 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = false;
     static constexpr int WAVES = 4;  // occupancy the registers are fitted to
     static constexpr int WAVES_KEPT = 4;
-    using Lds = Lds64Full;
+    template <int N>
+    using Lds = Lds64Full<N>;
     struct Tgt {
         double x, y, z, vx, vy, vz, rho;
     };
@@ -269,7 +290,8 @@ struct Continuity64 {
         r.a1 = COL(c, C_VX, q); r.a2 = COL(c, C_VY, q); r.a3 = COL(c, C_VZ, q);
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.zr[l] = make_double2(r.z, r.a0);
@@ -282,9 +304,9 @@ struct Continuity64 {
     static __device__ void fold(Acc& a, const Part& q) { a.d += q.d; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
+    template <int V, class L>
     static __device__ void interact(const PairCtx&, const SysDesc& d, const Tgt& t,
-                                    const Lds& s, int jj, int, int, double dx, double dy,
+                                    const L& s, int jj, int, int, double dx, double dy,
                                     double dz, double ss, Acc& a)
     {
         const double r = sqrt(ss);
@@ -308,9 +330,11 @@ struct Continuity64 {
 template <bool EMIT>
 struct Pairs64 {
     static constexpr bool INTERACT = EMIT;
+    static constexpr bool USES_Q = true;  // interact reads the candidate index
     static constexpr int WAVES = 5;  // occupancy the registers are fitted to
     static constexpr int WAVES_KEPT = 5;
-    using Lds = Lds64Pos;
+    template <int N>
+    using Lds = Lds64Pos<N>;
     struct Tgt {
         double x, y, z;
     };
@@ -329,7 +353,8 @@ struct Pairs64 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.z[l] = r.z;
@@ -340,8 +365,8 @@ struct Pairs64 {
     static __device__ void fold(Acc&, const Part&) {}
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const L&,
                                     int, int p, int q, double, double, double, double, Acc&)
     {
         const unsigned long long k = atomicAdd(c.emit_counter, 1ull);
@@ -356,9 +381,11 @@ struct Pairs64 {
 
 struct Digest64 {
     static constexpr bool INTERACT = true;
+    static constexpr bool USES_Q = true;  // interact reads the candidate index
     static constexpr int WAVES = 5;
     static constexpr int WAVES_KEPT = 5;
-    using Lds = Lds64Pos;
+    template <int N>
+    using Lds = Lds64Pos<N>;
     struct Tgt {
         double x, y, z;
     };
@@ -377,7 +404,8 @@ struct Digest64 {
         fetch_pos<P>(c, q, g, r);
         return r;
     }
-    static __device__ void put(const Raw& r, int l, Lds& s)
+    template <class L>
+    static __device__ void put(const Raw& r, int l, L& s)
     {
         s.xy[l] = make_double2(r.x, r.y);
         s.z[l] = r.z;
@@ -388,8 +416,8 @@ struct Digest64 {
     static __device__ void fold(Acc& a, const Part& q) { a.h += q.h; a.n += q.n; }
     static constexpr int NVAR = 1;
     static __device__ int variant(const SysDesc&) { return 0; }
-    template <int V>
-    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const Lds&,
+    template <int V, class L>
+    static __device__ void interact(const PairCtx& c, const SysDesc&, const Tgt&, const L&,
                                     int, int, int q, double, double, double, double, Acc& a)
     {
         a.h += mix64((unsigned long long)c.refidx[q] + 1ull);
@@ -403,9 +431,9 @@ struct Digest64 {
 };
 
 // A lane's run: cnt accepted slots of w (lowest first) against target ht.
-template <int V, bool FLAT, int T, class Op>
+template <int V, bool FLAT, int T, class Op, class L>
 __device__ inline void walk64(const PairCtx& c, const SysDesc& d, const typename Op::Tgt& ht,
-                              const typename Op::Lds& lds, const int* q, unsigned long long w,
+                              const L& lds, const int* q, unsigned long long w,
                               int cnt, int rb, int hp, typename Op::Part& part)
 {
     if constexpr (V < Op::NVAR) {
@@ -418,7 +446,8 @@ __device__ inline void walk64(const PairCtx& c, const SysDesc& d, const typename
             const double dz = FLAT ? 0.0 : ht.z - lds_z(lds, jj);
             double ss = dx * dx + dy * dy;
             if (!FLAT) ss = ss + dz * dz;
-            Op::template interact<V>(c, d, ht, lds, jj, hp, q[jj], dx, dy, dz, ss, part);
+            Op::template interact<V>(c, d, ht, lds, jj, hp, Op::USES_Q ? q[jj] : 0, dx, dy, dz, ss,
+                                     part);
         }
     }
 }
@@ -426,7 +455,8 @@ __device__ inline void walk64(const PairCtx& c, const SysDesc& d, const typename
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op, bool KEPT>
 __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool store_masks,
                               bool kept_ok,
-                              typename Op::Lds& lds, Cands<ROWS, PERIODIC>& L,
+                              typename Op::template Lds<ring_slots<KEPT>()>& lds,
+                              Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()>& L,
                               Bal<typename Op::Part>& bal, unsigned& npairs,
                               unsigned long long& ntests)
 {
@@ -451,6 +481,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
     // testing pass wrote; otherwise it streams the candidate list and culls
     // it again
     constexpr bool use_kept = KEPT;  // (the kernel took the tiles whose masks and lists are loaded)
+    constexpr int RS = ring_slots<KEPT>();
     const bool store_kept = !KEPT && store_masks && kept_ok;
     const long long lbase = mbase / T * 64;
     const int nk = use_kept ? ld_const(c.kept_n + tile) : 0;
@@ -475,8 +506,8 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
                 const int q = c.kept[lbase + k];
                 Seg sh{};
                 if constexpr (PERIODIC) sh = range_shift(L, c.kept_rg[lbase + k]);
-                Op::put(Op::template fetch<PERIODIC>(c, q, sh), k & 127, lds);
-                L.q[k & 127] = q;
+                Op::put(Op::template fetch<PERIODIC>(c, q, sh), k & (RS - 1), lds);
+                if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & (RS - 1)] = q;
             }
             fill = min(nk, head + 64);
             last = true;
@@ -492,7 +523,7 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
                                (unsigned)(rq - first) < (unsigned)nvalid);
             const int pos = compact(L, fill, nself, keep, rq, rrg, first);
             if (pos >= 0) {
-                Op::put(raw, pos & 127, lds);
+                Op::put(raw, pos & (RS - 1), lds);
                 if (store_kept) {
                     c.kept[lbase + pos] = rq;
                     if constexpr (PERIODIC) c.kept_rg[lbase + pos] = (unsigned char)rrg;
@@ -506,12 +537,12 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
                 // the last chunk: pad its free slots
                 const int k = head + lane;
                 if (k >= fill) {
-                    Op::put(pad_raw(), k & 127, lds);
-                    L.q[k & 127] = first;
+                    Op::put(pad_raw(), k & (RS - 1), lds);
+                    if constexpr (cands_q<Op, KEPT>() > 1) L.q[k & (RS - 1)] = first;
                 }
                 __syncthreads();
             }
-            const int rb = head & 64;  // ring half of the chunk
+            const int rb = head & (RS - 64);  // the chunk's part of the ring
             // accepted-slot word of target t (owner lanes publish it)
             unsigned long long word = 0;
             if (use_kept) {
@@ -602,8 +633,8 @@ __device__ inline void tile64(const PairCtx& c, int tile, bool load_masks, bool 
 template <int ROWS, bool FLAT, bool PERIODIC, int T, class Op, bool KEPT>
 __global__ __launch_bounds__(64, KEPT ? Op::WAVES_KEPT : Op::WAVES) void k_pair64(PairCtx c)
 {
-    __shared__ typename Op::Lds lds;
-    __shared__ Cands<ROWS, PERIODIC> L;
+    __shared__ typename Op::template Lds<ring_slots<KEPT>()> lds;
+    __shared__ Cands<ROWS, PERIODIC, cands_q<Op, KEPT>()> L;
     __shared__ Bal<typename Op::Part> bal;
     const int ntiles = ld_const(c.meta + META_NTILES);
     const bool ok = c.masks != nullptr && ld_const(c.meta + META_MASK_OK) != 0;

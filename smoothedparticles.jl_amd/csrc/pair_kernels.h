@@ -504,6 +504,24 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
     return tot;
 }
 
+// Slots of the LDS ring of staged candidates.  A testing pass appends the
+// kept candidates of each streamed step behind those not yet worked, so it
+// needs two chunks' room; the kept-list kernel stages one chunk per step and
+// works it before the next, so one chunk's room does (less LDS, more waves).
+template <bool KEPT>
+constexpr int ring_slots()
+{
+    return KEPT ? 64 : 128;
+}
+
+// Entries of the candidate-index ring: the kept-list kernel of an operator
+// that does not read the index keeps none (LDS for one more wave per SIMD).
+template <class Op, bool KEPT>
+constexpr int cands_q()
+{
+    return KEPT && !Op::USES_Q ? 1 : 128;
+}
+
 // ------------------------------------------------------------- staging --
 // A candidate's fields as loaded from the columns (the image shift already
 // added to the position).  The kernels load the next chunk's candidates into
