@@ -34,7 +34,7 @@ template <int OUT>
 struct Density64 {
     static constexpr bool INTERACT = true;
     static constexpr bool USES_Q = false;
-    static constexpr int WAVES = 5;  // occupancy the registers are fitted to
+    static constexpr int WAVES = 6;  // occupancy the registers are fitted to (80 VGPRs)
     static constexpr int WAVES_KEPT = 5;
     template <int N>
     using Lds = Lds64Pos<N>;
