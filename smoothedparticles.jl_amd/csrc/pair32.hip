@@ -17,8 +17,9 @@ namespace sph {
 // which pairs interact is decided by the predicate, not by r.
 __device__ inline float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// LDS ring of kept candidates (FP32): (x, y, z, |x|^2) -- (x, y, z,
-// P/rho^2) for the force pass -- and, for the force pass, (vx, vy, vz, rho).
+// LDS ring of kept candidates (FP32), N slots: (x, y, z, |x|^2) -- (x, y, z,
+// P/rho^2) for the force pass -- and, for the force pass, (vx, vy, vz,
+// rho * 0.5 / avk); the other operators stage their own fourth component.
 template <int N>
 struct Lds32Pos {
     float4 pos[N];

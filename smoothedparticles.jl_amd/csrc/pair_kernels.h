@@ -439,8 +439,8 @@ __device__ inline int runs_of(int pop, int B, int m)
 // Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
 // only, 0 elsewhere).  Returns the total.  When the run length is the
 // largest pop, every target is one run and the runs need no moving: own is
-// set and each owner lane walks its own target (tt, r0, cnt are then not
-// set).  That is always so when all 64 lanes hold a target with pairs, since
+// set and each owner lane walks its own target (tt, r0 and cnt are then
+// unused).  That is always so when all 64 lanes hold a target with pairs, since
 // a shorter run length splits the largest and needs 65 runs.  Otherwise
 // sets the lane's run (target tt, first rank r0, count cnt) and, for owner
 // lanes, the first run and the number of runs of their target.
