@@ -294,6 +294,38 @@ def test_removal_matches_reference_order():
     assert np.array_equal(g.get("x"), o.field("x"))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dense_cell_and_large_removal_bucket(precision):
+    """Buckets larger than the rank's LDS staging: a cell holding 3000
+    particles and a removal bucket of 2500, beside ordinary cells (the blocks
+    that straddle them rank in global memory).  Fields come back in the
+    reference's order, the neighbour digests and the density equal the
+    oracle's."""
+    rng = np.random.default_rng(97)
+    spread = rng.random((2000, 3))
+    dense = 0.5 + 0.05 * rng.random((3000, 3))
+    out = rng.random((2500, 3))
+    out[:, 0] = 2.0                       # outside the box: removed
+    x = np.concatenate([spread, dense, out])
+    x[:, 2] = 0.0
+    x = x[rng.permutation(len(x))]
+    lo, hi = np.zeros(3), np.array([1.0, 1.0, 0.0])
+    v, rho, P, typ = fields(rng, len(x), 2)
+    g, o = make_both(x, 0.05, lo, hi, precision, v, rho, P, typ)
+    assert sp.create_cell_list(g) == o.create_cell_list() == 5000
+    for f in ("x", "v", "rho", "type"):
+        assert np.array_equal(g.get(f), o.field(f)), f
+    cg, hg = sp.neighbour_digest(g)
+    co, ho = o.neighbour_digest()
+    assert np.array_equal(cg, co) and np.array_equal(hg, ho)
+    sp.apply(g, sp.reset_rho)
+    sp.apply(g, sp.find_rho, self=True, m=0.37)
+    o.reset_rho()
+    o.density_sum(True, m=0.37)
+    tol = 1e-12 if precision == "fp64" else 5e-6
+    assert np.max(np.abs(g.get("rho") / o.field("rho") - 1)) <= tol
+
+
 def test_stale_cell_list_is_an_error():
     x, lo, hi, _ = random_cloud(500, 2, 81)
     g, _ = make_both(x, 0.1, lo, hi)
