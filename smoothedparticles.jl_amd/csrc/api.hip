@@ -264,6 +264,7 @@ void free_cell_arrays(sph_system* s)
     const long long c = s->cells_cap;
     dfree(s, s->cnt, c); dfree(s, s->cell_start, c + 1); dfree(s, s->tile_start, c + 1);
     dfree(s, s->cell_chunks, c); dfree(s, s->mask_start, c + 1); dfree(s, s->cell_sys, c);
+    dfree(s, s->cell_order, c);
     dfree(s, s->scan_part, s->scan_part_n); dfree(s, s->scan_part64, s->scan_part_n);
 }
 
@@ -282,7 +283,7 @@ int ensure_cells(sph_system* s, long long buckets)
     if ((rc = dalloc(s, &s->cnt, ncap)) || (rc = dalloc(s, &s->cell_start, ncap + 1)) ||
         (rc = dalloc(s, &s->tile_start, ncap + 1)) || (rc = dalloc(s, &s->cell_chunks, ncap)) ||
         (rc = dalloc(s, &s->mask_start, ncap + 1)) || (rc = dalloc(s, &s->cell_sys, ncap)) ||
-        (rc = dalloc(s, &s->scan_part, s->scan_part_n)) ||
+        (rc = dalloc(s, &s->cell_order, ncap)) || (rc = dalloc(s, &s->scan_part, s->scan_part_n)) ||
         (rc = dalloc(s, &s->scan_part64, s->scan_part_n))) {
         s->cells_cap = 0;
         return rc;

@@ -161,31 +161,34 @@ struct SrcCount {
     const unsigned* c;
     __device__ V operator()(int i) const { return c[i]; }
 };
-// Tiles of a cell: ceil(count / T).
+// Tiles of the cell at brick rank i: ceil(count / T).
 struct SrcTiles {
     using V = unsigned;
     const int* cs;
+    const int* order;
     int ncells;
     int T;
     __device__ V operator()(int i) const
     {
         if (i >= ncells) return 0u;
-        const int n = cs[i + 1] - cs[i];
+        const int c = order[i];
+        const int n = cs[c + 1] - cs[c];
         return (unsigned)((n + T - 1) / T);
     }
 };
-// Mask words of a cell: tiles * chunks per target * T.
+// Mask words of the cell at brick rank i: tiles * chunks per target * T.
 struct SrcMaskWords {
     using V = unsigned long long;
     const int* tile_start;
     const int* chunks;
+    const int* order;
     int ncells;
     int T;
     __device__ V operator()(int i) const
     {
         if (i >= ncells) return 0ull;
         const long long nt = tile_start[i + 1] - tile_start[i];
-        return (V)(nt * chunks[i] * T);
+        return (V)(nt * chunks[order[i]] * T);
     }
 };
 
@@ -364,20 +367,47 @@ __global__ void k_remove_seq(const int* meta, int nsys, int cells_total, const i
     }
 }
 
+// Rank of local cell lc of a system in brick order: the grid is cut into
+// bricks of BX x BY x BZ cells (BZ = 1 in 2D), the bricks are numbered x
+// fastest, then y, then z, and the cells of a brick x fastest; the bricks
+// on the far faces are cut to the grid, so the ranks are 0 .. key_max - 1.
+// The tile table follows this order: each XCD walks a contiguous share of
+// it, and the tiles it has in flight then cover a compact block of cells
+// whose neighbour cells (the candidates) are few enough to stay in its L2,
+// where a row-major order spreads them over whole rows of the grid.
+constexpr int BRICK_X = 4, BRICK_Y = 4, BRICK_Z = 4, BRICK_2D = 8;
+
+__device__ inline int brick_rank(const Geo& g, int lc)
+{
+    const int bx = g.L3 > 1 ? BRICK_X : BRICK_2D;
+    const int by = g.L3 > 1 ? BRICK_Y : BRICK_2D;
+    const int bz = g.L3 > 1 ? BRICK_Z : 1;
+    const int ix = lc % g.L1, r = lc / g.L1;
+    const int iy = r % g.L2, iz = r / g.L2;
+    const int kx = ix / bx, ky = iy / by, kz = iz / bz;
+    const int wx = min(bx, g.L1 - kx * bx);  // extents of this brick
+    const int hy = min(by, g.L2 - ky * by);
+    const int hz = min(bz, g.L3 - kz * bz);
+    return kz * bz * g.L1 * g.L2 + ky * by * g.L1 * hz + kx * bx * hy * hz +
+           (ix - kx * bx) + wx * ((iy - ky * by) + hy * (iz - kz * bz));
+}
+
 // 64-candidate chunks one target of global cell c walks: the pair kernels
 // concatenate the cell's stencil ranges and cut the list into chunks of 64.
+// Also places c at its brick rank in order[].
 template <int ROWS>
 __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__ sysd,
                                                      const int* __restrict__ cell_sys,
                                                      int ncells, const int* __restrict__ cell_start,
-                                                     int* chunks)
+                                                     int* chunks, int* order)
 {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= ncells) return;
+    const SysDesc& d = sysd[cell_sys[c]];
+    const int lc = c - d.cell_base;
+    order[d.cell_base + brick_rank(d.g, lc)] = c;
     int n = 0;
     if (cell_start[c + 1] > cell_start[c]) {
-        const SysDesc& d = sysd[cell_sys[c]];
-        const int lc = c - d.cell_base;
         for (int row = 0; row < ROWS; ++row) {
             Seg seg[2];
             const int ns = row_segments<ROWS>(d, cell_start, lc, row, seg);
@@ -388,12 +418,13 @@ __global__ __launch_bounds__(256) void k_cell_chunks(const SysDesc* __restrict__
     chunks[c] = (n + 63) / 64;
 }
 
-// Tile table: tile k of cell c covers targets cell_start[c] + k*T ...; its
-// pair masks start at mask_start[c] + k * chunks[c] * T, its kept-candidate
-// list at 64 / T times that.
+// Tile table, in brick order: tile k of the cell c at rank r covers targets
+// cell_start[c] + k*T ...; its pair masks start at mask_start[r] +
+// k * chunks[c] * T, its kept-candidate list at 64 / T times that.
 __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_start,
                                                const int* __restrict__ tile_start,
                                                const int* __restrict__ chunks,
+                                               const int* __restrict__ order,
                                                const unsigned long long* __restrict__ mask_start,
                                                int ncells, int T, long long mask_cap,
                                                long long kept_cap, int4* tiles, int* meta,
@@ -413,13 +444,14 @@ __global__ __launch_bounds__(256) void k_tiles(const int* __restrict__ cell_star
     // cells with tiles: counted per block, one global atomic per block (a
     // single address serialises its atomics)
     int n = 0;
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) {
-        const int t0 = tile_start[c], nt = tile_start[c + 1] - t0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < ncells; r += gridDim.x * 256) {
+        const int t0 = tile_start[r], nt = tile_start[r + 1] - t0;
         n += nt > 0;
+        const int c = order[r];
         const int cs = cell_start[c];
         const long long words = (long long)chunks[c] * T;
         for (int t = 0; t < nt; ++t) {
-            const unsigned long long base = mask_start[c] + (unsigned long long)(t * words);
+            const unsigned long long base = mask_start[r] + (unsigned long long)(t * words);
             tiles[t0 + t] = make_int4(c, cs + t * T, (int)(unsigned)(base & 0xffffffffull),
                                       (int)(base >> 32));
         }
@@ -490,16 +522,18 @@ void enqueue_tiles(sph_system* s)
 {
     const int ncells = (int)s->cells_total;
     const unsigned nbc = nblocks(ncells + 1, 256);
-    scan(s, SrcTiles{s->cell_start, ncells, s->tile_T}, ncells + 1, s->tile_start, s->scan_part);
     if (s->rows == 3)
         k_cell_chunks<3><<<nbc, 256, 0, s->stream>>>(s->sysd, s->cell_sys, ncells, s->cell_start,
-                                                     s->cell_chunks);
+                                                     s->cell_chunks, s->cell_order);
     else
         k_cell_chunks<9><<<nbc, 256, 0, s->stream>>>(s->sysd, s->cell_sys, ncells, s->cell_start,
-                                                     s->cell_chunks);
-    scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, ncells, s->tile_T}, ncells + 1,
-         s->mask_start, s->scan_part64);
-    k_tiles<<<nbc < 256 ? nbc : 256, 256, 0, s->stream>>>(s->cell_start, s->tile_start, s->cell_chunks,
+                                                     s->cell_chunks, s->cell_order);
+    scan(s, SrcTiles{s->cell_start, s->cell_order, ncells, s->tile_T}, ncells + 1, s->tile_start,
+         s->scan_part);
+    scan(s, SrcMaskWords{s->tile_start, s->cell_chunks, s->cell_order, ncells, s->tile_T},
+         ncells + 1, s->mask_start, s->scan_part64);
+    k_tiles<<<nbc < 256 ? nbc : 256, 256, 0, s->stream>>>(s->cell_start, s->tile_start,
+                                        s->cell_chunks, s->cell_order,
                                         s->mask_start, ncells, s->tile_T,
                                         s->masks ? s->mask_cap : 0,
                                         s->kept && (s->kept_rg || !s->periodic) ? s->kept_cap : 0,

@@ -308,7 +308,7 @@ struct sph_system {
     // build scratch
     unsigned* cnt;       // buckets
     int* cell_start;     // buckets + 1
-    int* tile_start;     // cells + 1
+    int* tile_start;     // cells + 1, by brick rank (cell_order)
     unsigned* scan_part; // scan partials
     long long scan_part_n;
     int* slot;           // cap
@@ -320,7 +320,8 @@ struct sph_system {
     int4* tiles;         // tile bound
     long long tile_cap;
     int* cell_chunks;    // cells: 64-candidate chunks of a target of the cell
-    unsigned long long* mask_start;  // cells + 1: first mask word of each cell
+    int* cell_order;     // cells: global cell at each brick rank (tile-table order)
+    unsigned long long* mask_start;  // cells + 1, by brick rank: first mask word
     unsigned long long* scan_part64;
     unsigned long long* masks;       // pair masks of the current cell list
     int* kept;                       // kept-candidate lists of the tiles (64 per chunk)
