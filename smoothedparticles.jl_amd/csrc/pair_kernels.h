@@ -379,11 +379,17 @@ __device__ inline unsigned long long drop_low_bits(unsigned long long w, int n)
     return w & (~0ull << pos);
 }
 
-__device__ inline int wave_max(int v)
+// Largest v over the wave, 0 <= v < 128, bit by bit from the top: one
+// compare per bit, the result is uniform.
+__device__ inline int wave_max7(int v)
 {
+    int r = 0;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o));
-    return __builtin_amdgcn_readfirstlane(v);
+    for (int b = 6; b >= 0; --b) {
+        const int c = r | (1 << b);
+        if (__ballot(v >= c) != 0ull) r = c;
+    }
+    return r;
 }
 
 // Copy of v as held by lane src.
@@ -399,24 +405,25 @@ __device__ inline V shfl_value(const V& v, int src)
     return r;
 }
 
-// Wave sum of v, 0 <= v < 128, from bit-sliced ballots: no cross-lane
-// data movement through LDS, the result is uniform.
-__device__ inline int wave_sum7(int v)
+// Inclusive prefix sum of v over the wave's lanes, in six DPP steps: the
+// row shifts by 1, 2, 4 and 8 lanes scan each row of 16 lanes, then the
+// broadcast of lane 15 carries row 0's total into row 1 and row 2's into
+// row 3, and the broadcast of lane 31 carries rows 0-1's total into rows 2
+// and 3.  Lanes a step does not write add the 0 given as the old value.
+// Every lane of the wave must be active.
+__device__ inline int wave_scan(int v)
 {
-    int sum = 0;
-#pragma unroll
-    for (int b = 0; b < 7; ++b) sum += __popcll(__ballot((v >> b) & 1)) << b;
-    return sum;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
 }
 
-// Sum of v over the lanes below this one, 0 <= v < 128.
-__device__ inline int lanes_below_sum7(int v)
-{
-    int sum = 0;
-#pragma unroll
-    for (int b = 0; b < 7; ++b) sum += lane_rank(__ballot((v >> b) & 1)) << b;
-    return sum;
-}
+// Sum of v over the wave (uniform).
+__device__ inline int wave_total(int v) { return __builtin_amdgcn_readlane(wave_scan(v), 63); }
 
 // ceil(65536 / d) for a uniform 1 <= d <= 64: a reciprocal estimate, then
 // corrected to the exact value.
@@ -437,26 +444,30 @@ __device__ inline int runs_of(int pop, int B, int m)
 }
 
 // Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
-// only, 0 elsewhere).  Returns the total.  When the run length is the
-// largest pop, every target is one run and the runs need no moving: own is
-// set and each owner lane walks its own target (tt, r0 and cnt are then
-// unused).  That is always so when all 64 lanes hold a target with pairs, since
-// a shorter run length splits the largest and needs 65 runs.  Otherwise
-// sets the lane's run (target tt, first rank r0, count cnt) and, for owner
-// lanes, the first run and the number of runs of their target.
+// only, 0 elsewhere).  Returns 0 when no target has pairs, else a positive
+// value.  When the run length is the largest pop, every target is one run
+// and the runs need no moving: own is set and each owner lane walks its own
+// target (tt, r0 and cnt are then unused).  That is always so when all 64
+// lanes hold a target with pairs, since a shorter run length splits the
+// largest and needs 65 runs.  Otherwise sets the lane's run (target tt,
+// first rank r0, count cnt) and, for owner lanes, the first run and the
+// number of runs of their target.  The wave sums are DPP scans and the
+// comparisons with the largest pop are ballots (pop > B on some lane), so
+// the split moves no data through LDS until the runs are assigned.
 template <class Part>
 __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, int& tt, int& r0,
                                  int& cnt, int& first_run, int& nruns)
 {
-    const int tot = wave_sum7(pop);
+    const unsigned long long with_pairs = __ballot(pop > 0);
     own = false;
     tt = -1;
     r0 = cnt = first_run = nruns = 0;
-    if (tot == 0) return 0;
-    if (__ballot(pop > 0) == ~0ull) {
+    if (with_pairs == 0ull) return 0;
+    if (with_pairs == ~0ull) {
         own = true;
-        return tot;
+        return 1;
     }
+    const int tot = wave_total(pop);
     // The run length B: the mean (rounded up) or one more if the runs then
     // fit in 64 lanes; else at most two bisection steps between that and
     // ceil(tot / (64 - nz)) (nz: lanes with pairs), which always fits since
@@ -464,17 +475,16 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
     // B; at B >= max(pop) every target is one run.  A full search for the
     // smallest B costs more tests than it saves pairs on sparse chunks.
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
-    int need = wave_sum7(runs_of(pop, B, run_magic(B)));
-    const int maxp = wave_max(pop);
-    if (need > 64 && B < maxp) {
+    int need = wave_total(runs_of(pop, B, run_magic(B)));
+    if (need > 64 && __ballot(pop > B) != 0ull) {
         ++B;
-        need = wave_sum7(runs_of(pop, B, run_magic(B)));
-        if (need > 64 && B < maxp) {
-            const int nz = __popcll(__ballot(pop > 0));  // < 64: not every lane has pairs
-            int lo = B + 1, hi = min((tot + (64 - nz) - 1) / (64 - nz), maxp);
+        need = wave_total(runs_of(pop, B, run_magic(B)));
+        if (need > 64 && __ballot(pop > B) != 0ull) {
+            const int nz = __popcll(with_pairs);  // < 64: not every lane has pairs
+            int lo = B + 1, hi = min((tot + (64 - nz) - 1) / (64 - nz), wave_max7(pop));
             for (int it = 0; it < 2 && lo < hi; ++it) {
                 const int mid = (lo + hi) >> 1;
-                if (wave_sum7(runs_of(pop, mid, run_magic(mid))) > 64)
+                if (wave_total(runs_of(pop, mid, run_magic(mid))) > 64)
                     lo = mid + 1;
                 else
                     hi = mid;
@@ -482,16 +492,16 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
             B = hi;
         }
     }
-    if (B >= maxp) {
+    if (__ballot(pop > B) == 0ull) {  // B >= max(pop)
         own = true;
         return tot;
     }
     const int m = run_magic(B);
     nruns = runs_of(pop, B, m);
-    first_run = lanes_below_sum7(nruns);
+    first_run = wave_scan(nruns) - nruns;
     bal.lane_t[lane] = -1;
     __syncthreads();
-    for (int k = 0; k < nruns; ++k) bal.lane_t[first_run + k] = lane;
+    for (int k = 0; k < nruns && first_run + k < 64; ++k) bal.lane_t[first_run + k] = lane;
     __syncthreads();
     tt = bal.lane_t[lane];
     const int src = tt < 0 ? 0 : tt;
