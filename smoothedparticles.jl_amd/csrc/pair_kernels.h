@@ -425,23 +425,22 @@ __device__ inline int wave_scan(int v)
 // Sum of v over the wave (uniform).
 __device__ inline int wave_total(int v) { return __builtin_amdgcn_readlane(wave_scan(v), 63); }
 
-// ceil(65536 / d) for a uniform 1 <= d <= 64: a reciprocal estimate, then
-// corrected to the exact value.
-__device__ inline int run_magic(int d)
+// ceil(pop / B) for 0 <= pop <= 64 and a uniform 1 <= B <= 65, branch-free:
+// floor((pop + B - 1/2) / B), a value at least 1 / (2B) away from every
+// integer, computed as pop * (1/B) + (B - 1/2) / B with the hardware
+// reciprocal (1 ulp) and one FMA.  Its FP32 error stays below 3e-5 / B, so
+// the truncation gives the exact quotient.
+struct RunDiv {
+    float r, c;
+};
+
+__device__ inline RunDiv run_div(int B)
 {
-    int m = (int)(65536.0f * __builtin_amdgcn_rcpf((float)d)) + 1;
-    while ((m - 1) * d >= 65536) --m;
-    while (m * d < 65536) ++m;
-    return m;
+    const float r = __builtin_amdgcn_rcpf((float)B);
+    return {r, ((float)B - 0.5f) * r};
 }
 
-// ceil(pop / B) for 0 <= pop <= 64: with m = ceil(65536 / B) and B <= 64,
-// floor(n m / 65536) = floor(n / B) for n < 1024.
-__device__ inline int runs_of(int pop, int B, int m)
-{
-    if (pop <= 0) return 0;
-    return B <= 64 ? ((pop + B - 1) * m) >> 16 : (pop + B - 1) / B;
-}
+__device__ inline int runs_of(int pop, RunDiv d) { return (int)fmaf((float)pop, d.r, d.c); }
 
 // Runs of the chunk.  pop: accepted pairs of the lane's target (owner lanes
 // only, 0 elsewhere).  Returns 0 when no target has pairs, else a positive
@@ -475,16 +474,16 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
     // B; at B >= max(pop) every target is one run.  A full search for the
     // smallest B costs more tests than it saves pairs on sparse chunks.
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
-    int need = wave_total(runs_of(pop, B, run_magic(B)));
-    if (need > 64 && __ballot(pop > B) != 0ull) {
+    // the runs at B and at B + 1 in one scan (16-bit halves: each sum <= 4096)
+    const int both = wave_total(runs_of(pop, run_div(B)) | (runs_of(pop, run_div(B + 1)) << 16));
+    if ((both & 0xffff) > 64 && __ballot(pop > B) != 0ull) {
         ++B;
-        need = wave_total(runs_of(pop, B, run_magic(B)));
-        if (need > 64 && __ballot(pop > B) != 0ull) {
+        if ((both >> 16) > 64 && __ballot(pop > B) != 0ull) {
             const int nz = __popcll(with_pairs);  // < 64: not every lane has pairs
             int lo = B + 1, hi = min((tot + (64 - nz) - 1) / (64 - nz), wave_max7(pop));
             for (int it = 0; it < 2 && lo < hi; ++it) {
                 const int mid = (lo + hi) >> 1;
-                if (wave_total(runs_of(pop, mid, run_magic(mid))) > 64)
+                if (wave_total(runs_of(pop, run_div(mid))) > 64)
                     lo = mid + 1;
                 else
                     hi = mid;
@@ -496,8 +495,7 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
         own = true;
         return tot;
     }
-    const int m = run_magic(B);
-    nruns = runs_of(pop, B, m);
+    nruns = runs_of(pop, run_div(B));
     first_run = wave_scan(nruns) - nruns;
     bal.lane_t[lane] = -1;
     __syncthreads();
