@@ -347,7 +347,7 @@ __device__ inline float box_dist2(const Box<float>& b, float x, float y, float z
 template <class Part>
 struct Bal {
     unsigned long long w[64];  // accepted slots of each target
-    int lane_t[64];            // target of each lane's run (-1: none)
+    int mark[64];              // run starts: (target + 1, its pop) packed, else 0
     Part part[64];             // each lane's partial sums of the chunk
 };
 
@@ -425,6 +425,19 @@ __device__ inline int wave_scan(int v)
 // Sum of v over the wave (uniform).
 __device__ inline int wave_total(int v) { return __builtin_amdgcn_readlane(wave_scan(v), 63); }
 
+// Inclusive prefix maximum of v >= 0 over the wave's lanes: the steps of
+// wave_scan with max for the sum.
+__device__ inline int wave_max_scan(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+
 // ceil(pop / B) for 0 <= pop <= 64 and a uniform 1 <= B <= 65, branch-free:
 // floor((pop + B - 1/2) / B), a value at least 1 / (2B) away from every
 // integer, computed as pop * (1/B) + (B - 1/2) / B with the hardware
@@ -496,18 +509,24 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
         return tot;
     }
     nruns = runs_of(pop, run_div(B));
-    first_run = wave_scan(nruns) - nruns;
-    bal.lane_t[lane] = -1;
+    const int incl = wave_scan(nruns);
+    first_run = incl - nruns;
+    const int used = __builtin_amdgcn_readlane(incl, 63);  // <= 64
+    // The lane where a target's runs start gets a mark (target + 1, pop);
+    // a lane's run belongs to the last mark at or below it.  The mark, with
+    // its lane index added, goes through a prefix maximum: the target
+    // numbers grow with the start lanes, so the maximum is that last mark.
+    bal.mark[lane] = 0;
     __syncthreads();
-    for (int k = 0; k < nruns && first_run + k < 64; ++k) bal.lane_t[first_run + k] = lane;
+    if (nruns > 0 && first_run < 64) bal.mark[first_run] = ((lane + 1) << 16) | (pop << 8);
     __syncthreads();
-    tt = bal.lane_t[lane];
-    const int src = tt < 0 ? 0 : tt;
-    const int st = __shfl(first_run, src);
-    const int pt = __shfl(pop, src);
-    if (tt >= 0) {
-        r0 = (lane - st) * B;
-        cnt = min(B, pt - r0);
+    int mk = bal.mark[lane];
+    if (mk != 0) mk |= lane;
+    mk = wave_max_scan(mk);
+    if (lane < used) {
+        tt = (mk >> 16) - 1;
+        r0 = (lane - (mk & 0xff)) * B;
+        cnt = min(B, ((mk >> 8) & 0xff) - r0);
     }
     return tot;
 }
