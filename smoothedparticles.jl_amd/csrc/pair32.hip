@@ -92,7 +92,7 @@ struct Density32 {
         const float u2 = u * u;
         q.w = fmaf(u2 * u2, fmaf(4.0f * ih, r, 1.0f), q.w);
     }
-    static __device__ void fold(Acc& a, const Part& q) { a.s += (double)q.w; }
+    static __device__ void fold(Acc& a, const Part& q, const SysDesc&) { a.s += (double)q.w; }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int out_col)
     {
@@ -195,10 +195,11 @@ struct Force32 {
             q.lx = fmaf(u3, dvx, q.lx); q.ly = fmaf(u3, dvy, q.ly); q.lz = fmaf(u3, dvz, q.lz);
         }
     }
-    static __device__ void fold(Acc& a, const Part& q)
+    // the laminar sums only when the term is on (a uniform branch)
+    static __device__ void fold(Acc& a, const Part& q, const SysDesc& d)
     {
         a.ax += (double)q.ax; a.ay += (double)q.ay; a.az += (double)q.az;
-        a.lx += (double)q.lx; a.ly += (double)q.ly; a.lz += (double)q.lz;
+        if (d.ph.mu != 0.0) { a.lx += (double)q.lx; a.ly += (double)q.ly; a.lz += (double)q.lz; }
     }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
@@ -278,7 +279,7 @@ struct ForceRho032 {
         const float f = u * u * u * (t.P + s.pos[jj].w);
         q.ax = fmaf(f, dx, q.ax); q.ay = fmaf(f, dy, q.ay); q.az = fmaf(f, dz, q.az);
     }
-    static __device__ void fold(Acc& a, const Part& q)
+    static __device__ void fold(Acc& a, const Part& q, const SysDesc&)
     {
         a.ax += (double)q.ax; a.ay += (double)q.ay; a.az += (double)q.az;
     }
@@ -364,7 +365,7 @@ struct Continuity32 {
         const float dd = fmaf(2.0f * (float)d.ph.nu, t.dr - s.pos[jj].w, xv);
         q.d = fmaf(u * u * u, dd, q.d);
     }
-    static __device__ void fold(Acc& a, const Part& q) { a.s += (double)q.d; }
+    static __device__ void fold(Acc& a, const Part& q, const SysDesc&) { a.s += (double)q.d; }
     static __device__ void finish(const PairCtx& c, const SysDesc& d, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
     {
@@ -429,7 +430,7 @@ struct Pairs32 {
             c.pair_out[2 * k + 1] = c.refidx[q];
         }
     }
-    static __device__ void fold(Acc&, const Part&) {}
+    static __device__ void fold(Acc&, const Part&, const SysDesc&) {}
     static __device__ void finish(const PairCtx&, const SysDesc&, int, const Tgt&, const Acc&,
                                   int, bool, int) {}
 };
@@ -481,7 +482,7 @@ struct Digest32 {
         a.h += mix64((unsigned long long)c.refidx[q] + 1ull);
         a.n += 1;
     }
-    static __device__ void fold(Acc& a, const Part& q) { a.h += q.h; a.n += q.n; }
+    static __device__ void fold(Acc& a, const Part& q, const SysDesc&) { a.h += q.h; a.n += q.n; }
     static __device__ void finish(const PairCtx& c, const SysDesc&, int p, const Tgt&,
                                   const Acc& a, int T, bool write, int)
     {
@@ -700,7 +701,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                     default: walk32<3, FLAT, T, Op>(c, d, tg, lds, L.q, word, n, rb, hp, part); break;
                     }
                 }
-                if (n > 0) Op::fold(acc, part);
+                if (n > 0) Op::fold(acc, part, d);
             } else if (tot > 0) {
                 typename Op::Part part;
                 Op::zero(part);
@@ -720,7 +721,7 @@ __device__ inline void tile32(const PairCtx& c, int tile, int out_col, bool load
                 }
                 bal.part[lane] = part;
                 __syncthreads();
-                for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k]);
+                for (int k = 0; k < nrun; ++k) Op::fold(acc, bal.part[frun + k], d);
             }
             npairs += (unsigned)__popcll(word);
             if (store_masks && s == 0) c.masks[mbase + (long long)chunk * T + t] = word;
