@@ -358,24 +358,26 @@ __device__ inline int slot_of_bit(int b)
     return (b % T) * (64 / T) + b / T;
 }
 
-// w without its n lowest set bits (0 <= n < popcount(w)).
+// w without its n lowest set bits (0 <= n < popcount(w)): the n-th set bit
+// is found in the 32-bit half that holds it, halving the field each step.
 __device__ inline unsigned long long drop_low_bits(unsigned long long w, int n)
 {
     if (n <= 0) return w;
-    unsigned long long x = w;
-    int pos = 0;
-    int c = __popc((unsigned)x);
-    if (n >= c) { n -= c; x >>= 32; pos += 32; }
-    c = __popc((unsigned)x & 0xffffu);
+    const unsigned lo = (unsigned)w;
+    const int clo = __popc(lo);
+    const bool up = n >= clo;
+    unsigned x = up ? (unsigned)(w >> 32) : lo;
+    n = up ? n - clo : n;
+    int pos = up ? 32 : 0;
+    int c = __popc(x & 0xffffu);
     if (n >= c) { n -= c; x >>= 16; pos += 16; }
-    c = __popc((unsigned)x & 0xffu);
+    c = __popc(x & 0xffu);
     if (n >= c) { n -= c; x >>= 8; pos += 8; }
-    c = __popc((unsigned)x & 0xfu);
+    c = __popc(x & 0xfu);
     if (n >= c) { n -= c; x >>= 4; pos += 4; }
-    c = __popc((unsigned)x & 0x3u);
+    c = __popc(x & 0x3u);
     if (n >= c) { n -= c; x >>= 2; pos += 2; }
-    c = (int)((unsigned)x & 1u);
-    if (n >= c) pos += 1;
+    if (n >= (int)(x & 1u)) pos += 1;
     return w & (~0ull << pos);
 }
 
