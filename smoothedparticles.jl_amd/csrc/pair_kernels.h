@@ -442,18 +442,28 @@ __device__ inline int wave_max_scan(int v)
 
 // ceil(pop / B) for 0 <= pop <= 64 and a uniform 1 <= B <= 65, branch-free:
 // floor((pop + B - 1/2) / B), a value at least 1 / (2B) away from every
-// integer, computed as pop * (1/B) + (B - 1/2) / B with the hardware
-// reciprocal (1 ulp) and one FMA.  Its FP32 error stays below 3e-5 / B, so
-// the truncation gives the exact quotient.
+// integer, computed as pop * (1/B) + (B - 1/2) / B with one FMA.  The two
+// factors are rounded to FP32 at compile time and read from a table with a
+// scalar load (B is uniform); the FMA's error stays below 3e-5 / B, so the
+// truncation gives the exact quotient.
 struct RunDiv {
     float r, c;
 };
 
-__device__ inline RunDiv run_div(int B)
-{
-    const float r = __builtin_amdgcn_rcpf((float)B);
-    return {r, ((float)B - 0.5f) * r};
-}
+#define SPH_RUN_DIV(b) {1.0f / (float)(b), ((float)(b) - 0.5f) / (float)(b)}
+#define SPH_RUN_DIV8(b)                                                                        \
+    SPH_RUN_DIV(b), SPH_RUN_DIV(b + 1), SPH_RUN_DIV(b + 2), SPH_RUN_DIV(b + 3),                \
+        SPH_RUN_DIV(b + 4), SPH_RUN_DIV(b + 5), SPH_RUN_DIV(b + 6), SPH_RUN_DIV(b + 7)
+static __constant__ RunDiv kRunDiv[72] = {{0.0f, 0.0f},     SPH_RUN_DIV(1),     SPH_RUN_DIV(2),
+                                          SPH_RUN_DIV(3),   SPH_RUN_DIV(4),     SPH_RUN_DIV(5),
+                                          SPH_RUN_DIV(6),   SPH_RUN_DIV(7),     SPH_RUN_DIV8(8),
+                                          SPH_RUN_DIV8(16), SPH_RUN_DIV8(24),   SPH_RUN_DIV8(32),
+                                          SPH_RUN_DIV8(40), SPH_RUN_DIV8(48),   SPH_RUN_DIV8(56),
+                                          SPH_RUN_DIV8(64)};
+#undef SPH_RUN_DIV8
+#undef SPH_RUN_DIV
+
+__device__ inline RunDiv run_div(int B) { return kRunDiv[B]; }
 
 __device__ inline int runs_of(int pop, RunDiv d) { return (int)fmaf((float)pop, d.r, d.c); }
 
@@ -489,8 +499,13 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
     // B; at B >= max(pop) every target is one run.  A full search for the
     // smallest B costs more tests than it saves pairs on sparse chunks.
     int B = (tot + 63) >> 6;  // <= 64: pop <= 64 on each lane
-    // the runs at B and at B + 1 in one scan (16-bit halves: each sum <= 4096)
-    const int both = wave_total(runs_of(pop, run_div(B)) | (runs_of(pop, run_div(B + 1)) << 16));
+    const int B0 = B;
+    // the runs at B and at B + 1 in one scan (16-bit halves: each sum <= 4096,
+    // so the low half never carries into the high one); its prefixes serve the
+    // run assignment below when B ends at one of the two
+    const int n0 = runs_of(pop, run_div(B)), n1 = runs_of(pop, run_div(B + 1));
+    const int scan01 = wave_scan(n0 | (n1 << 16));
+    const int both = __builtin_amdgcn_readlane(scan01, 63);
     if ((both & 0xffff) > 64 && __ballot(pop > B) != 0ull) {
         ++B;
         if ((both >> 16) > 64 && __ballot(pop > B) != 0ull) {
@@ -510,8 +525,17 @@ __device__ inline int split_runs(int lane, int pop, Bal<Part>& bal, bool& own, i
         own = true;
         return tot;
     }
-    nruns = runs_of(pop, run_div(B));
-    const int incl = wave_scan(nruns);
+    int incl;
+    if (B == B0) {
+        nruns = n0;
+        incl = scan01 & 0xffff;
+    } else if (B == B0 + 1) {
+        nruns = n1;
+        incl = scan01 >> 16;
+    } else {
+        nruns = runs_of(pop, run_div(B));
+        incl = wave_scan(nruns);
+    }
     first_run = incl - nruns;
     const int used = __builtin_amdgcn_readlane(incl, 63);  // <= 64
     // The lane where a target's runs start gets a mark (target + 1, pop);
