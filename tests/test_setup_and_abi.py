@@ -176,3 +176,22 @@ def test_jitter_draws_are_counter_based():
                 u = float(mix((seed + (t + 1) * gamma) & M) >> 11) * 2.0 ** -53
                 ref[i, d] = 0.0 + (2.0 * u - 1.0) * amp
         assert np.array_equal(x, ref)
+
+
+def test_run_count_division_is_exact():
+    """The pair kernels' run split counts a target's runs as ceil(pop / B)
+    with one FMA, trunc(pop * r + c), r = 1/B and c = (B - 1/2)/B rounded to
+    FP32 (a compile-time table in csrc/pair_kernels.h, run_div / runs_of).
+    Exact for every pair count pop <= 64 and run length B <= 71, also with r
+    one ulp off either way (the margin is 1/(2B), the error below 3e-5/B).
+    The FMA's single rounding is emulated by the exact double sum (under 40
+    significant bits here) rounded to FP32."""
+    pops = np.arange(65, dtype=np.float64)
+    for B in range(1, 72):
+        r = np.float32(1.0) / np.float32(B)
+        c = (np.float32(B) - np.float32(0.5)) / np.float32(B)
+        for rr in (r, np.nextafter(r, np.float32(0)), np.nextafter(r, np.float32(1))):
+            v = (pops * np.float64(rr) + np.float64(c)).astype(np.float32)
+            got = np.trunc(v).astype(np.int64)
+            want = (np.arange(65) + B - 1) // B
+            assert np.array_equal(got, want), (B, rr)
