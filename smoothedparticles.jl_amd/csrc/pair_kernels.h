@@ -327,16 +327,19 @@ __device__ inline float box_dist2(const Box<float>& b, float x, float y, float z
 
 // ------------------------------------------------------ balanced pairs --
 // The accepted pairs of a chunk are spread over all 64 lanes instead of
-// being walked by the lane that owns the target: with ~15% acceptance the
-// per-lane counts of a chunk differ a lot (targets near the chunk's
-// particles accept many, far ones none), and an owner-walked loop runs as
-// long as the busiest lane.  Per chunk:
+// being walked by the lane that owns the target: the per-lane counts of a
+// chunk differ a lot (targets near the chunk's candidates accept many, far
+// ones none), and an owner-walked loop runs as long as the busiest lane.
+// Per chunk:
 //   - the owner lane of target t (slice 0, lane t) publishes the target's
 //     64-bit word of accepted slots in LDS;
 //   - the pairs are cut into runs of at most B pairs, one target per run,
 //     with B a quota from the mean up for which the runs fit in 64 lanes
 //     (or, when every target is one run, each owner lane walks its own);
 //     lane l takes run l;
+//   - the run split is wave-uniform work (DPP scans, ballots, a table
+//     lookup), and a lane finds its run from a packed mark and a prefix
+//     maximum (split_runs);
 //   - a lane reads its target's data from the owner lane (shuffles), skips
 //     to its run with a popcount descent and walks the run;
 //   - its partial sums go to LDS, and the owner folds the runs of its target
@@ -577,10 +580,11 @@ constexpr int cands_q()
 
 // ------------------------------------------------------------- staging --
 // A candidate's fields as loaded from the columns (the image shift already
-// added to the position).  The kernels load the next chunk's candidates into
-// registers before working on the current chunk and write them to LDS at
-// the next iteration, so the loads' latency overlaps the tests and pairs.
-// Each operator fills (fetch) and writes (put) only the fields it uses.
+// added to the position).  A kernel streams its candidates 64 at a time:
+// each lane loads one into registers, and the kept ones are converted and
+// written to the LDS ring in the same step (the other waves on the SIMD
+// cover the loads' latency).  Each operator fills (fetch) and writes (put)
+// only the fields it uses.
 struct Raw {
     double x, y, z, a0, a1, a2, a3, a4;
 };
