@@ -19,7 +19,7 @@
 using namespace sph;
 
 namespace sph {
-void enqueue_build(sph_system* s);
+void enqueue_build(sph_system* s, unsigned skip_cols = 0);
 void enqueue_tiles(sph_system* s);
 void enqueue_unary(sph_system* s, int op, double* prho2);
 void enqueue_prho2(sph_system* s);
@@ -824,7 +824,10 @@ int ds_step(sph_system* s, int eos, long long tiles_bound)
     int rc;
     hipEvent_t ev[2];
     prof_begin(s, ev);
-    enqueue_build(s);
+    // the density pass overwrites rho of every particle in a cell and the
+    // EOS then P of every particle, before anything reads them: the build
+    // need not carry the two columns over
+    enqueue_build(s, (1u << C_RHO) | (1u << C_P));
     prof_end(s, SPH_KERNEL_BUILD, ev);
     if ((rc = run_pair(s, P_DENSITY, 1, 1, s->pair_counter, tiles_bound, SPH_KERNEL_DENSITY,
                        MASK_TEST_STORE, 0, s->pair_counter + 2)))

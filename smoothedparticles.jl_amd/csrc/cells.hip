@@ -302,14 +302,16 @@ __global__ __launch_bounds__(256) void k_gather(const int* meta, const int* __re
                                                 const double* __restrict__ src,
                                                 double* __restrict__ dst,
                                                 const int* __restrict__ isrc,
-                                                int* __restrict__ idst, long long cap)
+                                                int* __restrict__ idst, long long cap,
+                                                unsigned skip)
 {
     const int n = meta[META_NTOT];
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     const long long j = perm[k];
 #pragma unroll
-    for (int f = 0; f < NCOL; ++f) dst[f * cap + k] = src[f * cap + j];
+    for (int f = 0; f < NCOL; ++f)
+        if (!((skip >> f) & 1u)) dst[f * cap + k] = src[f * cap + j];
 #pragma unroll
     for (int f = 0; f < NICOL; ++f) idst[f * cap + k] = isrc[f * cap + j];
 }
@@ -547,7 +549,9 @@ void enqueue_tiles(sph_system* s)
                                       c + C_Y * cap, c + C_Z * cap, s->tile_T, s->boxes);
 }
 
-void enqueue_build(sph_system* s)
+// skip_cols: columns (bit C_*) the caller rewrites for every particle before
+// reading them; the gather leaves their values in the new order undefined.
+void enqueue_build(sph_system* s, unsigned skip_cols)
 {
     hipStream_t st = s->stream;
     const long long cap = s->cap;
@@ -568,7 +572,7 @@ void enqueue_build(sph_system* s)
     k_rank<<<nb, 256, 0, st>>>(s->meta, s->cell_start, s->tmp_idx, s->tmp_key, s->tmp_ref,
                                s->perm, ncells + nsys);
     k_gather<<<nb, 256, 0, st>>>(s->meta, s->perm, c, s->cols[s->cur ^ 1], ic,
-                                 s->icols[s->cur ^ 1], cap);
+                                 s->icols[s->cur ^ 1], cap, skip_cols);
     s->cur ^= 1;
     int* ref = s->icols[s->cur] + I_REF * cap;
     int* sysid = s->icols[s->cur] + I_SYS * cap;
